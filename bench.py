@@ -1,0 +1,144 @@
+"""Benchmark: nonlocal stiffness assembly on one MI355X.
+
+Workload (default): unit square h=1/256 (131,072 P1 triangles, 66,049 vertices), delta=0.05,
+constant kernel, retriangulate with caps, 7-point outer/inner rules, CG, FP64.  A step is one
+complete assembly of the matrix with the mesh already resident in HBM: dual graph, BFS
+neighbour lists, integer sparsity pattern, pair quadrature, compaction to CSR.
+
+Prints one JSON line: element-pairs/s (value), ms per assembled matrix, the roofline of the
+dominant kernel (algorithmic FP64 flops / its average launch time, measured with HIP events on
+the library's stream) and the CPU baseline (the FP64 OpenMP oracle on a bounded sample of the
+same workload, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix are equal on this part)
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (n, delta, kernel dict, config kwargs, jitter)
+    "h256_delta0.05_constant": (256, 0.05, {"function": "constant", "horizon": 0.05, "outputdim": 1}, {}, 0.0),
+    "h128_delta0.1_parabola_jitter0.2": (128, 0.1, {"function": "parabola", "horizon": 0.1, "outputdim": 1}, {}, 0.2),
+    "h64_delta0.2_fractional_s0.4": (64, 0.2, {"function": "fractional", "horizon": 0.2, "outputdim": 1,
+                                               "fractional_s": 0.4},
+                                     {"close": "fractional", "tensorGaussDegree": 5}, 0.0),
+    "h16_delta0.1_constant": (16, 0.1, {"function": "constant", "horizon": 0.1, "outputdim": 1}, {}, 0.0),
+}
+DEFAULT = "h256_delta0.05_constant"
+
+
+def algorithmic_flops(st, nPx, nPy):
+    """FP64 flop count of the pair quadrature: 150 per outer point (toPhys, retriangulation,
+    outer block update) and 40 per inner point (toPhys, pull-back, basis, weight, 4-block
+    accumulate).  Interior pairs have one sub-triangle per outer point; cut pairs report the
+    sub-triangles they actually integrate."""
+    f_int = st["interior"] * (nPx * 150 + nPx * nPy * 40)
+    f_cut = st["cut"] * nPx * 150 + st["subtri"] * nPy * 40
+    return f_int, f_cut
+
+
+def cpu_baseline(mesh, kernel, conf, budget_s=15.0):
+    """FP64 OpenMP oracle on a sample of outer elements (centre of the mesh), all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nE = mesh["nE"]
+    cores = os.cpu_count() or 1
+    s = max(64, min(nE, 4 * cores))
+    # calibrate, then size the sample to the time budget
+    while True:
+        b = nE // 2 - s // 2
+        r = oracle.par_system(mesh, kernel, conf, outer=(b, b + s))
+        if r["seconds"] > 1.0 or s >= nE:
+            break
+        s = min(nE, s * 4)
+    target = int(s * budget_s / max(r["seconds"], 1e-3))
+    if target > s * 1.5:
+        s = min(nE, target)
+        b = nE // 2 - s // 2
+        r = oracle.par_system(mesh, kernel, conf, outer=(b, b + s))
+    return {"value": r["n_pairs"] / r["seconds"], "unit": "element-pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{s} of {nE} outer elements (centre of the mesh), {r['n_pairs']} pairs, "
+                      f"{r['seconds']:.2f} s incl. dual graph and merge, OMP threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
+        sys.exit(2)
+
+    from nlfem_amd import nlfem
+    from nlfem_amd.mesh import unit_square
+    from nlfem_amd.config import make_configuration
+
+    n, delta, kernel, ckw, jitter = WORKLOADS[args.workload]
+    mesh = unit_square(n, delta, jitter=jitter, seed=2024)
+    conf = make_configuration(**ckw)
+    conf["precision"] = args.precision
+    print(f"[bench] workload={args.workload} nE={mesh['nE']} building plan", file=sys.stderr, flush=True)
+    asm = nlfem.Assembler(mesh, kernel, conf)
+    for i in range(args.warmup):
+        asm.assemble()
+        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+    ms_phase = {}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        asm.assemble()
+        st = asm.stats()
+        for k, v in st.items():
+            if k.startswith("ms_"):
+                ms_phase[k] = ms_phase.get(k, 0.0) + v
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / args.steps
+    for k in ms_phase:
+        ms_phase[k] /= args.steps
+    pairs = st["pairs"]
+    value = pairs / (ms_step / 1000.0)
+    nPx = conf["quadrature"]["outer"]["points"].shape[0]
+    nPy = conf["quadrature"]["inner"]["points"].shape[0]
+    f_int, f_cut = algorithmic_flops(st, nPx, nPy)
+    dom = "cut" if ms_phase["ms_cut"] >= ms_phase["ms_interior"] else "interior"
+    f_dom = f_cut if dom == "cut" else f_int
+    t_dom = ms_phase["ms_" + dom] / 1000.0
+    achieved = f_dom / t_dom / 1e12
+    roof = {"bound": "mfma", "compute": "fp64 (VALU; FP64 vector and matrix peaks coincide on MI355X)",
+            "kernel": f"k_pairs<{'cut' if dom == 'cut' else 'interior'}>", "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "traffic": None, "flops_per_launch": f_dom, "ms_per_launch": round(t_dom * 1000, 4)}
+    out = {
+        "metric": "element-pairs/s (assembled ordered element pairs per second, full assembly)",
+        "value": value, "unit": "element-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-sum/f64-accumulate", "data": "synthetic",
+        "config": {"workload": args.workload, "nE": mesh["nE"], "nV": mesh["nV"], "delta": delta,
+                   "kernel": kernel["function"], "method": "retriangulate+caps", "quadrature": f"{nPx}/{nPy}",
+                   "precision": args.precision},
+        "pairs": int(pairs), "probes": int(st["probes"]), "nnz": int(st["nnz"]),
+        "phases_ms": {k[3:]: round(v, 3) for k, v in ms_phase.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(mesh, kernel, conf, args.cpu_budget)
+    asm.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

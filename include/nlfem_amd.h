@@ -1,0 +1,108 @@
+/* nlfem_amd.h -- C ABI of the MI355X nonlocal stiffness assembly (libnlfem_amd.so).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a nonzero status
+ * on failure; nlfem_last_error() then describes the failure (thread-local).  Host pointers
+ * are borrowed for the duration of the call, read-only unless documented as output.
+ * The library allocates its device memory itself (hipMalloc) and runs on the current HIP
+ * device; every plan owns one non-blocking stream with default priority.
+ *
+ * Reference interfaces (cagau/nlfem) replaced by each entry point are cited per function.
+ */
+#ifndef NLFEM_AMD_H
+#define NLFEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes */
+#define NLFEM_OK 0
+#define NLFEM_EINVAL 1   /* invalid configuration; the reference would abort() */
+#define NLFEM_EDEVICE 2  /* HIP runtime / device failure */
+#define NLFEM_ECAPACITY 3 /* a capacity estimate could not be met */
+#define NLFEM_EIO 4      /* result file could not be written */
+
+const char* nlfem_last_error(void);
+
+/* Drop-in for par_assemble (include/Cassemble.h:125-140; Cython binding
+ * include/Cassemble.pxd:12-25).  compute = "system" writes the stiffness matrix A^T in
+ * Armadillo ARMA_SPM_BIN_FN008 format to path_spAd (the reference's output contract,
+ * src/Cassemble.cpp:850-851); "forcing" writes the load vector as ARMA_MAT_BIN_FN008 to
+ * path_fd (src/Cassemble.cpp:927).  std::string arguments become NUL-terminated strings.
+ * K_Omega, nE_Omega (checked), nV_Omega, is_NeumannBoundary, str_model_f (system),
+ * maxDiameter and verbose keep the reference's (lack of) effect on values.  Only dim == 2 and
+ * nZeta == 0 are supported (NLFEM_EINVAL otherwise). */
+int nlfem_par_assemble(const char* compute, const char* path_spAd, const char* path_fd, int K_Omega, int K,
+                       const long* ptrTriangles, const long* ptrLabelTriangles, const double* ptrVerts,
+                       const long* ptrLabelVerts, int nE, int nE_Omega, int nV, int nV_Omega, const double* Px,
+                       int nPx, const double* dx, const double* Py, int nPy, const double* dy, double sqdelta,
+                       int is_DiscontinuousGalerkin, int is_NeumannBoundary, const char* str_model_kernel,
+                       const char* str_model_f, const char* str_integration_method_remote,
+                       const char* str_integration_method_close, int is_PlacePointOnCap, int dim, int outdim,
+                       const long* ptrZetaIndicator_indices, const long* ptrZetaIndicator_indptr, long nZeta,
+                       const double* Pg, int degree, const double* dg, double maxDiameter, double fractional_s,
+                       int is_fullConnectedComponentSearch, int verbose);
+
+/* Drop-in for par_evaluateMass (include/Cassemble.h:202-204): vd += M ud over the first J
+ * elements (the reference's Python caller passes J = nE_Omega, cython/nlfem.pyx:674). */
+int nlfem_par_evaluateMass(double* vd, const double* ud, const long* Elements, const long* ElementLabels,
+                           const double* Verts, const long* VertexLabels, int K_Omega, int J, int nP, const double* P,
+                           const double* dx, int dim, int outdim, int is_DiscontinuousGalerkin, int nE, int nV);
+
+/* Debug helpers, as exported to Cython (include/Cassemble.pxd:30-38; include/Cassemble.h:62-71).
+ * They run the device implementation used by the assembly kernels. */
+int nlfem_method_retriangulate(const double* xCenter, const double* TE, double sqdelta, double* reTriangleList,
+                               int isPlacePointOnCap); /* returns #triangles (>= 0) or -status */
+int nlfem_toRef(const double* E, const double* phys_x, double* ref_p);
+int nlfem_toPhys(const double* E, const double* p, int dim, double* out_x);
+int nlfem_solve2x2(const double* A, const double* b, double* x);
+
+/* Batched variant of method_retriangulate: n centres x[2n], triangles TE[6n]; lists[54n]
+ * (9 triangles x 3 points x 2 coordinates each), ntri[n].  No counterpart in the reference;
+ * used to check the device predicate against CPU results in bulk. */
+int nlfem_retriangulate_batch(int n, const double* x, const double* TE, double sqdelta, int isPlacePointOnCap,
+                              double* lists, int* ntri);
+
+/* ------------------------------------------------------------------------------------
+ * Device-resident plan API (no counterpart in the reference, whose result crosses back
+ * through a file).  create uploads the mesh and configuration; assemble runs the whole
+ * pipeline on the GPU (dual graph, BFS neighbour lists, integer sparsity pattern, pair
+ * quadrature, compaction); the CSR of A stays in HBM until copied out.  precision: 0 = FP64,
+ * 1 = FP32 quadrature sums with FP64 accumulation and FP64 geometry.
+ * ------------------------------------------------------------------------------------ */
+typedef struct nlfem_plan nlfem_plan;
+
+int nlfem_plan_create(nlfem_plan** out, int K, const long* ptrTriangles, const long* ptrLabelTriangles,
+                      const double* ptrVerts, const long* ptrLabelVerts, int nE, int nV, const double* Px, int nPx,
+                      const double* dx, const double* Py, int nPy, const double* dy, double sqdelta,
+                      int is_DiscontinuousGalerkin, const char* str_model_kernel,
+                      const char* str_integration_method_remote, const char* str_integration_method_close,
+                      int is_PlacePointOnCap, int outdim, const double* Pg, int degree, const double* dg,
+                      double fractional_s, int is_fullConnectedComponentSearch, int precision);
+int nlfem_plan_assemble(nlfem_plan* plan);
+/* K (matrix order) and nnz of the assembled CSR of A (rows sorted, exact zeros dropped). */
+int nlfem_plan_csr_info(const nlfem_plan* plan, long long* K, long long* nnz);
+/* Copies rowptr[K+1], col[nnz], val[nnz] to host memory. */
+int nlfem_plan_csr_copy(const nlfem_plan* plan, long long* rowptr, int* col, double* val);
+/* Device pointers of the CSR (valid until the next assemble or destroy). */
+int nlfem_plan_csr_device(const nlfem_plan* plan, long long** rowptr, int** col, double** val);
+/* Neighbour lists: total number of (aT, bT) pairs, then ptr[nE+1], nbr[total], cls[total]
+ * (0 cut, 1 interior, 2 close, 3 baryCenter).  Order within a list is unspecified. */
+int nlfem_plan_neighbours_total(const nlfem_plan* plan, long long* total);
+int nlfem_plan_neighbours(const nlfem_plan* plan, long long* ptr, int* nbr, unsigned char* cls);
+/* Statistics: out[0..] = pairs, probes, close, interior, cut, bary, nnz_nodes, nnz, cap,
+ * max_list, subtri (fan triangles integrated by the cut kernel), then per-phase milliseconds (dual, bfs, transpose, pattern, positions, close,
+ * interior, cut, compact).  n = number of doubles available in out; returns NLFEM_OK. */
+int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
+void nlfem_plan_destroy(nlfem_plan* plan);
+
+/* Load vector on the device (par_forcing, src/Cassemble.cpp:857-928): fd[K]. */
+int nlfem_load_vector(int K, const long* ptrTriangles, const long* ptrLabelTriangles, const double* ptrVerts,
+                      const long* ptrLabelVerts, int nE, int nV, const double* Px, int nPx, const double* dx,
+                      int is_DiscontinuousGalerkin, const char* str_model_f, int outdim, double* fd);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLFEM_AMD_H */

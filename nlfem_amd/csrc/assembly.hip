@@ -1,0 +1,1548 @@
+// Nonlocal stiffness assembly on one MI355X (gfx950): device kernels and the host
+// orchestration of one plan.
+//
+// Pipeline (every phase stream-ordered on the plan's stream):
+//   setup      element records (coordinates + |det| + label, 64 B per element)
+//   dual       vertex->element incidence and the vertex-sharing dual graph
+//   bfs        one wavefront per outer element: level-synchronous BFS over the dual graph
+//              with an LDS hash set as the visited set; interaction predicates are the
+//              reference's FP64 retriangulation / barycentre tests, bit-exact.  Output is a
+//              padded neighbour list per outer element split into a front segment
+//              (interior + close pairs) and a back segment (cut pairs), so the quadrature
+//              kernels see uniform pair classes.
+//   transpose  inverse neighbour relation (pairs grouped by inner element)
+//   pattern    node-row sparsity pattern of the full symmetric matrix, built in integers
+//              (count pass, scan, fill pass with an LDS bitonic sort per row)
+//   positions  transpose-position map and per-element 3x3 position tables
+//   close      tensor-Gauss Duffy quadrature of first-layer pairs (singular kernels)
+//   interior   pairs whose inner element lies in every ball B(x_k, delta)
+//   cut        pairs cut by the ball boundary: streamed retriangulation per outer point
+//   compact    drop exact zeros (and unlabelled rows, which receive no contributions),
+//              expand od x od blocks, write the scalar CSR
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nlfem_device.hpp"
+#include "plan.hpp"
+
+namespace nlfem {
+
+#define HIPCHECK(x)                                                                                          \
+    do {                                                                                                     \
+        hipError_t e_ = (x);                                                                                 \
+        if (e_ != hipSuccess)                                                                                \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " __FILE__ ":" + \
+                                     std::to_string(__LINE__) + " (" #x ")");                                \
+    } while (0)
+
+constexpr int CNT_PAIRS = 0, CNT_PROBES = 1, CNT_CLOSE = 2, CNT_INTERIOR = 3, CNT_CUT = 4, CNT_BARY = 5,
+              CNT_OVERFLOW = 6, CNT_ERROR = 7, CNT_MAXLIST = 8, CNT_MAXROW = 9, CNT_MAXT = 10, CNT_SUBTRI = 11;
+
+// ======================================================================= small utilities
+__global__ void k_fill_int(int* p, int v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Exclusive scan of n counts into n+1 offsets: block-local scan (1024 items per block of
+// 256 threads), recursive scan of block sums, offset add.
+template <typename TIn>
+__global__ __launch_bounds__(256) void k_scan_block(const TIn* in, long long* out, long long* bsum, long long n) {
+    __shared__ long long s[256];
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x * 4;
+    long long v[4], loc = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = (base + i < n) ? (long long)in[base + i] : 0;
+        loc += v[i];
+    }
+    s[threadIdx.x] = loc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        long long t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long ex = s[threadIdx.x] - loc;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(long long* out, const long long* boff, long long n) {
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x * 4;
+    const long long o = boff[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (base + i < n) out[base + i] += o;
+}
+
+__global__ void k_set_last(long long* out, long long n, const long long* src) { out[n] = *src; }
+
+// scratch must hold 2*ceil(n/1024)+2 + (recursion) entries; Plan::scan_tmp is sized for that.
+template <typename TIn>
+static void scan_exclusive(const TIn* in, long long* out, long long n, long long* scratch, hipStream_t st) {
+    long long nb = (n + 1023) / 1024;
+    if (nb == 0) {
+        HIPCHECK(hipMemsetAsync(out, 0, sizeof(long long), st));
+        return;
+    }
+    long long* bsum = scratch;
+    long long* boff = scratch + nb;
+    k_scan_block<TIn><<<(unsigned)nb, 256, 0, st>>>(in, out, bsum, n);
+    HIPCHECK(hipGetLastError());
+    if (nb == 1) {
+        k_set_last<<<1, 1, 0, st>>>(out, n, bsum);
+        return;
+    }
+    scan_exclusive<long long>(bsum, boff, nb, boff + nb + 1, st);
+    k_scan_add<<<(unsigned)nb, 256, 0, st>>>(out, boff, n);
+    k_set_last<<<1, 1, 0, st>>>(out, n, boff + nb);
+    HIPCHECK(hipGetLastError());
+}
+
+static unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// LDS hash set (open addressing, linear probing, empty = -1).  Returns 1 if key was
+// inserted, 0 if present, -1 if the table is full.
+NLFEM_DEV int lds_insert(int* h, int H, int key) {
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
+    for (int p = 0; p < H; p++) {
+        int old = atomicCAS(&h[s], -1, key);
+        if (old == -1) return 1;
+        if (old == key) return 0;
+        s = (s + 1) & (uint32_t)(H - 1);
+    }
+    return -1;
+}
+NLFEM_DEV int lds_slot(int* h, int H, int key) {  // slot of key, inserting it if absent; -1 if full
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
+    for (int p = 0; p < H; p++) {
+        int old = atomicCAS(&h[s], -1, key);
+        if (old == -1 || old == key) return (int)s;
+        s = (s + 1) & (uint32_t)(H - 1);
+    }
+    return -1;
+}
+NLFEM_DEV int lds_find(const int* h, int H, int key) {
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
+    for (int p = 0; p < H; p++) {
+        int k = h[s];
+        if (k == key) return (int)s;
+        if (k == -1) return -1;
+        s = (s + 1) & (uint32_t)(H - 1);
+    }
+    return -1;
+}
+
+// Node (degree of freedom) j of element T: the vertex for CG, 3T+j for DG.
+NLFEM_DEV int node_of(const int* tri, int T, int j, int is_DG) { return is_DG ? 3 * T + j : tri[3 * T + j]; }
+NLFEM_DEV bool row_labelled(const int* vlabel, int node, int is_DG) { return is_DG || vlabel[node] > 0; }
+
+// ======================================================================= setup
+__global__ void k_setup(int nE, const long* tri64, const long* lab64, const double* verts, int* tri, ElemRec* elem,
+                        unsigned long long* counters, int nV) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    ElemRec r;
+    bool bad = false;
+    for (int j = 0; j < 3; j++) {
+        long v = tri64[3 * (long)e + j];
+        if (v < 0 || v >= nV) { bad = true; v = 0; }
+        tri[3 * e + j] = (int)v;
+        r.E[2 * j] = verts[2 * v];
+        r.E[2 * j + 1] = verts[2 * v + 1];
+    }
+    r.absDet = absDet(r.E);
+    r.label = (int)lab64[e];
+    r.pad = 0;
+    elem[e] = r;
+    if (bad) atomicOr(&counters[CNT_ERROR], 1ull);
+}
+
+__global__ void k_vlabel(int nV, const long* lab64, int* vlabel) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nV) vlabel[v] = (int)lab64[v];
+}
+
+// ======================================================================= dual graph
+__global__ void k_vinc_count(int nE, const int* tri, int* vcnt) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    for (int j = 0; j < 3; j++) atomicAdd(&vcnt[tri[3 * e + j]], 1);
+}
+__global__ void k_vinc_fill(int nE, const int* tri, const long long* vptr, int* vfill, int* vel) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    for (int j = 0; j < 3; j++) {
+        int v = tri[3 * e + j];
+        int s = atomicAdd(&vfill[v], 1);
+        vel[vptr[v] + s] = e;
+    }
+}
+__global__ void k_sort_vinc(int nV, const long long* vptr, int* vel) {  // sort incidence rows (insertion sort)
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    long long b = vptr[v], e = vptr[v + 1];
+    for (long long i = b + 1; i < e; i++) {
+        int x = vel[i];
+        long long j = i - 1;
+        while (j >= b && vel[j] > x) { vel[j + 1] = vel[j]; j--; }
+        vel[j + 1] = x;
+    }
+}
+
+constexpr int MAX_DUAL = 128;
+// Dual graph row of element e: elements sharing >= 1 vertex, self excluded, ascending
+// (replaces METIS_MeshToDual(ncommon=1), src/Cassemble.cpp:389-417).
+template <bool FILL>
+__global__ void k_dual(int nE, const int* tri, const long long* vptr, const int* vel, int* acnt,
+                       const long long* xadj, int* adj, unsigned long long* counters) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    int buf[MAX_DUAL];
+    int n = 0;
+    bool over = false;
+    for (int j = 0; j < 3; j++) {
+        int v = tri[3 * e + j];
+        for (long long p = vptr[v]; p < vptr[v + 1]; p++) {
+            int f = vel[p];
+            if (f == e) continue;
+            // insert f into the sorted buffer if absent
+            int i = n - 1;
+            bool dup = false;
+            for (int q = 0; q < n; q++)
+                if (buf[q] == f) { dup = true; break; }
+            if (dup) continue;
+            if (n >= MAX_DUAL) { over = true; continue; }
+            while (i >= 0 && buf[i] > f) { buf[i + 1] = buf[i]; i--; }
+            buf[i + 1] = f;
+            n++;
+        }
+    }
+    if (over) atomicOr(&counters[CNT_ERROR], 2ull);
+    if (!FILL) {
+        acnt[e] = n;
+    } else {
+        long long o = xadj[e];
+        for (int q = 0; q < n; q++) adj[o + q] = buf[q];
+    }
+}
+
+// ======================================================================= BFS
+struct BfsArgs {
+    int nE;
+    const ElemRec* elem;
+    const long long* xadj;
+    const int* adj;
+    QuadArgs q;
+    double sqdelta, sq_in;
+    int caps, remote, level0, fullCC;
+    int cap, H, F;
+    uint32_t* nbr;
+    int* cntF;
+    int* cntB;
+    int2* closeList;
+    long long closeCap;
+    unsigned long long* counters;
+};
+
+// Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
+// Returns the pair class or -1 when the pair does not interact.
+NLFEM_DEV int classify(const BfsArgs& a, const ElemRec& A, const ElemRec& B, int method) {
+    if (method == 2) return (int)CLS_CLOSE;
+    if (method == M_BARYCENTER) {
+        for (int k = 0; k < a.q.nPx; k++) {
+            double x[2];
+            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+            if (bary_interacts(x[0], x[1], B.E, a.sqdelta)) return 3;  // baryCenter pair
+        }
+        return a.fullCC ? 3 : -1;
+    }
+    bool interior = true;
+    for (int k = 0; k < a.q.nPx && interior; k++) {
+        double x[2];
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        for (int j = 0; j < 3; j++)
+            if (!(sqdist(B.E[2 * j], B.E[2 * j + 1], x[0], x[1]) <= a.sq_in)) interior = false;
+    }
+    if (interior) return (int)CLS_INTERIOR;
+    for (int k = 0; k < a.q.nPx; k++) {
+        double x[2];
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        NoEmit ne;
+        if (retriangulate<true>(x[0], x[1], B.E, a.sqdelta, a.caps != 0, ne) >= 3) return (int)CLS_CUT;
+    }
+    return a.fullCC ? (int)CLS_CUT : -1;
+}
+
+// One wavefront (one block of 64 lanes) per outer element.  Equivalent to the reference's
+// queue BFS (src/Cassemble.cpp:593-783): every element adjacent to an expanded element is
+// tested exactly once; interacting and label-0 elements are expanded; the first layer
+// {aT} u adj(aT) uses the close method.  The resulting set does not depend on visit order.
+__global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
+    extern __shared__ int lds[];
+    int* hash = lds;
+    int* test = hash + a.H;
+    int* next = test + a.F;
+    __shared__ int nTest, nNext, nTot, nF, nB, ovf;
+    __shared__ unsigned long long st[6];
+    const int lane = threadIdx.x;
+    const int aT = blockIdx.x;
+    if (aT >= a.nE) return;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) {
+        if (lane == 0) { a.cntF[aT] = 0; a.cntB[aT] = 0; }
+        return;
+    }
+    for (int i = lane; i < a.H; i += 64) hash[i] = -1;
+    if (lane == 0) {
+        nTest = 0; nNext = 0; nTot = 0; nF = 0; nB = 0; ovf = 0;
+        for (int i = 0; i < 6; i++) st[i] = 0;
+    }
+    __syncthreads();
+    // level 0 candidates: aT itself, then its dual-graph neighbours
+    const long long x0 = a.xadj[aT];
+    const int deg = (int)(a.xadj[aT + 1] - x0);
+    if (1 + deg > a.F) {
+        if (lane == 0) ovf = 1;
+    } else {
+        if (lane == 0) { lds_insert(hash, a.H, aT); test[0] = aT; }
+        __syncthreads();
+        for (int j = lane; j < deg; j += 64) {
+            int b = a.adj[x0 + j];
+            if (lds_insert(hash, a.H, b) < 0) ovf = 1;
+            test[1 + j] = b;
+        }
+        if (lane == 0) nTest = 1 + deg;
+    }
+    __syncthreads();
+    int level = 0;
+    while (!ovf) {
+        // test the candidates of this level
+        const int method = (level == 0) ? a.level0 : a.remote;
+        for (int j = lane; j < nTest; j += 64) {
+            const int b = test[j];
+            const ElemRec B = a.elem[b];
+            bool push = false;
+            if (B.label == 0) {
+                push = true;  // label-0 elements are traversed only (src/Cassemble.cpp:773-775)
+            } else {
+                int cls = classify(a, A, B, method);
+                if (cls >= 0) {
+                    push = true;
+                    int t = atomicAdd(&nTot, 1);
+                    if (t >= a.cap) {
+                        ovf = 1;
+                    } else {
+                        int slot;
+                        if (cls == (int)CLS_INTERIOR || cls == (int)CLS_CLOSE) slot = atomicAdd(&nF, 1);
+                        else slot = a.cap - 1 - atomicAdd(&nB, 1);
+                        uint32_t code = (cls == 3) ? CLS_CUT : (uint32_t)cls;
+                        a.nbr[(size_t)aT * a.cap + slot] = (uint32_t)b | (code << CLS_SHIFT) | (cls == 3 ? (1u << 31) : 0u);
+                    }
+                    atomicAdd(&st[cls == 3 ? 5 : (cls == (int)CLS_CLOSE ? 2 : (cls == (int)CLS_INTERIOR ? 3 : 4))], 1ull);
+                    if (cls == (int)CLS_CLOSE) {
+                        unsigned long long c = atomicAdd(&a.counters[CNT_CLOSE], 1ull);
+                        if ((long long)c < a.closeCap) a.closeList[c] = make_int2(aT, b);
+                    }
+                } else {
+                    atomicAdd(&st[1], 1ull);
+                }
+            }
+            if (push && b != aT) {
+                int q = atomicAdd(&nNext, 1);
+                if (q < a.F) next[q] = b;
+                else ovf = 1;
+            }
+        }
+        __syncthreads();
+        if (nNext == 0 || ovf) break;
+        if (lane == 0) nTest = 0;
+        __syncthreads();
+        // discover the next level: unvisited neighbours of the expanded elements
+        const int nn = nNext;
+        for (int j = lane; j < nn; j += 64) {
+            const int s = next[j];
+            for (long long p = a.xadj[s]; p < a.xadj[s + 1]; p++) {
+                int e = a.adj[p];
+                int r = lds_insert(hash, a.H, e);
+                if (r < 0) { ovf = 1; break; }
+                if (r == 1) {
+                    int q = atomicAdd(&nTest, 1);
+                    if (q < a.F) test[q] = e;
+                    else ovf = 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (lane == 0) nNext = 0;
+        __syncthreads();
+        level++;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        if (ovf) {
+            atomicOr(&a.counters[CNT_OVERFLOW], 1ull);
+            a.cntF[aT] = 0;
+            a.cntB[aT] = 0;
+        } else {
+            a.cntF[aT] = nF;
+            a.cntB[aT] = nB;
+            atomicMax(&a.counters[CNT_MAXLIST], (unsigned long long)(nF + nB));
+        }
+        atomicAdd(&a.counters[CNT_PAIRS], st[2] + st[3] + st[4] + st[5]);
+        atomicAdd(&a.counters[CNT_PROBES], st[1]);
+        atomicAdd(&a.counters[CNT_INTERIOR], st[3]);
+        atomicAdd(&a.counters[CNT_CUT], st[4]);
+        atomicAdd(&a.counters[CNT_BARY], st[5]);
+    }
+}
+
+NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
+
+// ======================================================================= transpose
+__global__ void k_tcount(int nE, int cap, const uint32_t* nbr, const int* cntF, const int* cntB, int* tcnt) {
+    int aT = blockIdx.x;
+    int n = cntF[aT] + cntB[aT];
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        uint32_t e = nbr[(size_t)aT * cap + list_index(p, cntF[aT], cap)];
+        atomicAdd(&tcnt[e & IDX_MASK], 1);
+    }
+}
+__global__ void k_tfill(int nE, int cap, const uint32_t* nbr, const int* cntF, const int* cntB,
+                        const long long* tptr, int* tfill, int* tadj) {
+    int aT = blockIdx.x;
+    int n = cntF[aT] + cntB[aT];
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        uint32_t e = nbr[(size_t)aT * cap + list_index(p, cntF[aT], cap)];
+        int b = (int)(e & IDX_MASK);
+        int s = atomicAdd(&tfill[b], 1);
+        tadj[tptr[b] + s] = aT;
+    }
+}
+
+// ======================================================================= pattern
+struct PatArgs {
+    int nNodes, is_DG, cap;
+    const int* tri;
+    const long long* vptr;
+    const int* vel;
+    const uint32_t* nbr;
+    const int* cntF;
+    const int* cntB;
+    const long long* tptr;
+    const int* tadj;
+    int EH, DH, SB;  // element hash, dof hash, sort buffer sizes (powers of two)
+    int* rowcnt;
+    const long long* rowptr;
+    int* col;
+    unsigned long long* counters;
+};
+
+// Row r of the full pattern = union over elements T holding node r, and over
+// U in N(T) u N^T(T), of nodes(T) u nodes(U).  One block of 256 threads per node row.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
+    extern __shared__ int lds[];
+    int* eh = lds;
+    int* dh = eh + a.EH;
+    int* sb = dh + a.DH;
+    __shared__ int nDof, ovf;
+    const int r = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.EH; i += 256) eh[i] = -1;
+    for (int i = tid; i < a.DH; i += 256) dh[i] = -1;
+    if (tid == 0) { nDof = 0; ovf = 0; }
+    __syncthreads();
+    long long t0, t1;
+    if (a.is_DG) { t0 = r / 3; t1 = t0 + 1; }
+    else { t0 = a.vptr[r]; t1 = a.vptr[r + 1]; }
+    for (long long ti = t0; ti < t1; ti++) {
+        const int T = a.is_DG ? (int)ti : a.vel[ti];
+        const int nF = a.cntF[T], n = nF + a.cntB[T];
+        const long long tb = a.tptr[T], te = a.tptr[T + 1];
+        const int nt = (int)(te - tb);
+        if (n + nt > 0 && tid == 0) { if (lds_insert(eh, a.EH, T) < 0) ovf = 1; }
+        for (int p = tid; p < n + nt; p += 256) {
+            int U;
+            if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
+            else U = a.tadj[tb + (p - n)];
+            if (lds_insert(eh, a.EH, U) < 0) ovf = 1;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < a.EH; i += 256) {
+        int U = eh[i];
+        if (U < 0) continue;
+        for (int j = 0; j < 3; j++) {
+            int d = node_of(a.tri, U, j, a.is_DG);
+            int rr = lds_insert(dh, a.DH, d);
+            if (rr < 0) ovf = 1;
+            else if (rr == 1) atomicAdd(&nDof, 1);
+        }
+    }
+    __syncthreads();
+    if (ovf) {
+        if (tid == 0) atomicOr(&a.counters[CNT_OVERFLOW], 2ull);
+        if (!FILL && tid == 0) a.rowcnt[r] = 0;
+        return;
+    }
+    const int n = nDof;
+    if (!FILL) {
+        if (tid == 0) {
+            a.rowcnt[r] = n;
+            atomicMax(&a.counters[CNT_MAXROW], (unsigned long long)n);
+        }
+        return;
+    }
+    if (n > a.SB) {
+        if (tid == 0) atomicOr(&a.counters[CNT_OVERFLOW], 4ull);
+        return;
+    }
+    // compact the dof hash into the sort buffer, pad to a power of two, bitonic sort
+    __shared__ int fillc;
+    if (tid == 0) fillc = 0;
+    __syncthreads();
+    for (int i = tid; i < a.DH; i += 256) {
+        int d = dh[i];
+        if (d >= 0) sb[atomicAdd(&fillc, 1)] = d;
+    }
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = n + tid; i < m; i += 256) sb[i] = 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < m; i += 256) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int x = sb[i], y = sb[ixj];
+                    bool up = ((i & k) == 0);
+                    if ((x > y) == up) { sb[i] = y; sb[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const long long o = a.rowptr[r];
+    for (int i = tid; i < n; i += 256) a.col[o + i] = sb[i];
+}
+
+// Position of column c in node row r (binary search in the sorted row); -1 if absent.
+NLFEM_DEV long long find_pos(const long long* rowptr, const int* col, int r, int c) {
+    long long lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo <= hi) {
+        long long mid = (lo + hi) >> 1;
+        int v = col[mid];
+        if (v == c) return mid;
+        if (v < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+__global__ void k_row_of(int nNodes, const long long* rowptr, int* rowOf) {
+    int r = blockIdx.x;
+    for (long long p = rowptr[r] + threadIdx.x; p < rowptr[r + 1]; p += blockDim.x) rowOf[p] = r;
+}
+__global__ void k_tpos(long long nnz, const long long* rowptr, const int* col, const int* rowOf, int* tpos,
+                       unsigned long long* counters) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    long long q = find_pos(rowptr, col, col[p], rowOf[p]);
+    if (q < 0) atomicOr(&counters[CNT_ERROR], 4ull);
+    tpos[p] = (int)q;
+}
+__global__ void k_posTT(int nE, const int* tri, int is_DG, const long long* rowptr, const int* col, int* posTT,
+                        unsigned long long* counters) {
+    int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (T >= nE) return;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            long long q = find_pos(rowptr, col, node_of(tri, T, i, is_DG), node_of(tri, T, j, is_DG));
+            posTT[9 * T + 3 * i + j] = (int)q;  // -1 when T takes part in no pair
+        }
+}
+
+// ======================================================================= pair quadrature
+struct NumArgs {
+    int nE, cap, is_DG, caps, HN, od;
+    const ElemRec* elem;
+    const int* tri;
+    const int* vlabel;
+    const uint32_t* nbr;
+    const int* cntF;
+    const int* cntB;
+    const long long* rowptr;
+    const int* col;
+    const int* tpos;
+    const int* posTT;
+    double* val;
+    QuadArgs q;
+    KernelParams kp;
+    unsigned long long* counters;
+};
+
+// Kernel value(s) at z = x - y: OO entries (row-major od x od).
+template <int KER, int OO, typename T>
+NLFEM_DEV void kernel_eval(T z0, T z1, const KernelParams& kp, T* K) {
+    if constexpr (KER == KER_MICRO_FIELD) {
+        T r = sqrt(z0 * z0 + z1 * z1);
+        T cd = (T)kp.c * ((T)1.0 / (r * r * r));
+        K[0] = cd * z0 * z0;
+        K[1] = cd * z0 * z1;
+        K[2] = cd * z1 * z0;
+        K[3] = cd * z1 * z1;
+    } else {
+        if constexpr (KER == KER_CONSTANT) K[0] = (T)kp.c;
+        else if constexpr (KER == KER_PARABOLA) K[0] = (T)kp.c * ((T)kp.sqdelta - (z0 * z0 + z1 * z1));
+        else if constexpr (KER == KER_FRACTIONAL) K[0] = (T)kp.c * ((T)1.0 / pow(sqrt(z0 * z0 + z1 * z1), (T)kp.exponent));
+        else K[0] = (T)kp.c * ((T)1.0 / sqrt(z0 * z0 + z1 * z1));
+    }
+}
+
+// Accumulators of one pair.  tN is termNonloc; for symmetric kernels termNonlocPrime equals
+// termNonloc bit for bit (k(y,x) is computed from the same |z| / z z^T products), so it is not
+// stored.  tLP is termLocalPrime.  tLacc accumulates termLocal over all pairs of a thread.
+template <int OD>
+struct PairBlocks {
+    static constexpr int OO = OD * OD, NB = 9 * OO;
+    double tN[NB];
+    double tLP[NB];
+};
+
+// Contribution of one inner quadrature point y (with weight w = dy_i * |det rT|, basis values
+// psi at y, kernel K = k(x_k, y)) to the inner sums of outer point k, and to termLocalPrime.
+// Follows src/integration.cpp:424-476.
+template <int OD, typename T>
+NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double wk, double* inL, double* inN, double* tLP) {
+#pragma clang fp contract(fast)
+    constexpr int OO = OD * OD;
+    T kw[OO];
+#pragma unroll
+    for (int o = 0; o < OO; o++) {
+        kw[o] = K[o] * w;
+        inL[o] += (double)kw[o];
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int o = 0; o < OO; o++) inN[b * OO + o] += (double)(psi[b] * kw[o]);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            T pp = psi[a] * psi[b];
+#pragma unroll
+            for (int o = 0; o < OO; o++)
+                tLP[a * OO * 3 + (o / OD) * OD * 3 + b * OD + o % OD] += (double)(pp * kw[o]) * wk;
+        }
+}
+
+// Outer accumulation after outer point k (src/integration.cpp:508-537):
+// termLocal += |aT| psi_a(x) psi_b(x) dx_k innerLocal, termNonloc += |aT| psi_a(x) dx_k innerNonloc.
+template <int OD>
+NLFEM_DEV void outer_point(const double* psx, double ak, const double* inL, const double* inN, double* tL,
+                           double* tN) {
+#pragma clang fp contract(fast)
+    constexpr int OO = OD * OD, N = 3 * OD;
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            tL[N * a + b] += ak * psx[a / OD] * psx[b / OD] * inL[OD * (a % OD) + (b % OD)];
+            tN[N * a + b] += ak * psx[a / OD] * inN[(a % OD) * OD + OO * (b / OD) + (b % OD)];
+        }
+}
+
+// Interior pair: the retriangulation of B(x_k,delta) n bT is bT itself for every k, so
+// the inner points are bT(Py_i) with basis values psiy_i (src/integration.cpp:383-538 with
+// Rdx = 1 and reTriangle = bT).
+template <int KER, int OD, typename T>
+NLFEM_DEV void pair_interior(const NumArgs& a, const ElemRec& A, const ElemRec& B, double* tL, PairBlocks<OD>& pb) {
+    constexpr int OO = OD * OD;
+    for (int i = 0; i < 9 * OO; i++) { pb.tN[i] = 0; pb.tLP[i] = 0; }
+    const double absA = A.absDet, absB = B.absDet;
+    for (int k = 0; k < a.q.nPx; k++) {
+        double x[2];
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        double inL[OO], inN[3 * OO];
+        for (int o = 0; o < OO; o++) inL[o] = 0;
+        for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+        const double wk = a.q.dx[k] * absA;
+        for (int i = 0; i < a.q.nPy; i++) {
+            double y[2];
+            toPhys(B.E, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+            T K[OO];
+            kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+            T psi[3] = {(T)a.q.psiy[3 * i], (T)a.q.psiy[3 * i + 1], (T)a.q.psiy[3 * i + 2]};
+            inner_point<OD, T>(K, (T)(a.q.dy[i] * absB), psi, wk, inL, inN, pb.tLP);
+        }
+        double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+        outer_point<OD>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
+    }
+}
+
+// General (cut) pair: streamed retriangulation per outer point, inner rule on every fan
+// triangle, basis values by the inverse affine map of bT (src/integration.cpp:383-538).
+template <int KER, int OD, typename T>
+NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, double* tL, PairBlocks<OD>& pb,
+                        unsigned& ntri) {
+    constexpr int OO = OD * OD;
+    for (int i = 0; i < 9 * OO; i++) { pb.tN[i] = 0; pb.tLP[i] = 0; }
+    const double absA = A.absDet;
+    const InvMap im = invmap(B.E);
+    for (int k = 0; k < a.q.nPx; k++) {
+        double x[2];
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        double inL[OO], inN[3 * OO];
+        for (int o = 0; o < OO; o++) inL[o] = 0;
+        for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+        const double wk = a.q.dx[k] * absA;
+        auto emit = [&](double r0, double r1, double s0, double s1, double t0, double t1) {
+            const double rT[6] = {r0, r1, s0, s1, t0, t1};
+            const double rTdet = absDet(rT);
+            ntri++;
+            for (int i = 0; i < a.q.nPy; i++) {
+                double y[2];
+                toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                T K[OO];
+                kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+                const double d0 = y[0] - im.e0, d1 = y[1] - im.e1;
+                const double p0 = im.i00 * d0 + im.i01 * d1;
+                const double p1 = im.i10 * d0 + im.i11 * d1;
+                T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
+                inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, wk, inL, inN, pb.tLP);
+            }
+        };
+        retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, emit);
+        double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+        outer_point<OD>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
+    }
+}
+
+
+// baryCenter pair (src/integration.cpp:910-1018 with method_baryCenter :1262-1280): the whole
+// inner element counts when its barycentre is in B(x_k, delta); factor 2 on termLocal and
+// termNonloc, the Prime blocks stay zero.
+template <int KER, int OD, typename T>
+NLFEM_DEV void pair_bary(const NumArgs& a, const ElemRec& A, const ElemRec& B, double* tL, PairBlocks<OD>& pb) {
+    constexpr int OO = OD * OD;
+    for (int i = 0; i < 9 * OO; i++) { pb.tN[i] = 0; pb.tLP[i] = 0; }
+    const double absA = A.absDet, absB = B.absDet;
+    double dummy[9 * OO];
+    for (int k = 0; k < a.q.nPx; k++) {
+        double x[2];
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        double inL[OO], inN[3 * OO];
+        for (int o = 0; o < OO; o++) inL[o] = 0;
+        for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+        if (bary_interacts(x[0], x[1], B.E, a.kp.sqdelta)) {
+            for (int i = 0; i < a.q.nPy; i++) {
+                double y[2];
+                toPhys(B.E, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                T K[OO];
+                kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+                T psi[3] = {(T)a.q.psiy[3 * i], (T)a.q.psiy[3 * i + 1], (T)a.q.psiy[3 * i + 2]};
+                inner_point<OD, T>(K, (T)(a.q.dy[i] * absB), psi, 0.0, inL, inN, dummy);
+            }
+        }
+        double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+        outer_point<OD>(psx, 2 * absA * a.q.dx[k], inL, inN, tL, pb.tN);
+    }
+}
+
+NLFEM_DEV double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+
+// Scatter of the blocks of pair (aT, b) (src/Cassemble.cpp:721-770) into the node-CSR
+// values with FP64 atomics.  Positions of (row rA[i], col node) come from the LDS map of the
+// three rows of aT, transposed entries through tpos, inner-element entries through posTT.
+template <int OD>
+NLFEM_DEV void scatter_pair(const NumArgs& a, const int* rA, const bool* labA, int b, bool prime,
+                            const PairBlocks<OD>& pb, const int* hk, const int* hv, bool useHash) {
+    constexpr int OO = OD * OD, N = 3 * OD;
+    int rB[3];
+    bool labB[3];
+    long long pos[3][3];  // pos[i][j]: entry (rA[i], rB[j])
+    for (int j = 0; j < 3; j++) {
+        rB[j] = node_of(a.tri, b, j, a.is_DG);
+        labB[j] = row_labelled(a.vlabel, rB[j], a.is_DG);
+    }
+    for (int j = 0; j < 3; j++) {
+        int s = useHash ? lds_find(hk, a.HN, rB[j]) : -1;
+        for (int i = 0; i < 3; i++) {
+            long long p;
+            if (useHash) p = (s >= 0) ? a.rowptr[rA[i]] + hv[3 * s + i] : -1;
+            else p = find_pos(a.rowptr, a.col, rA[i], rB[j]);
+            pos[i][j] = p;
+            if (p < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            const long long p = pos[i][j];
+            if (p < 0) continue;
+            if (labA[i]) {  // -termNonloc into (aAdx[i], bAdx[j])
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&a.val[p * OO + ao * OD + bo], -pb.tN[N * (i * OD + ao) + j * OD + bo]);
+            }
+            if (prime && labB[j]) {  // -termNonlocPrime into (bAdx[j], aAdx[i])
+                const long long q = a.tpos[p];
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&a.val[q * OO + bo * OD + ao], -pb.tN[N * (i * OD + ao) + j * OD + bo]);
+            }
+        }
+    }
+    if (prime) {  // +termLocalPrime into (bAdx[j], bAdx[i])
+        for (int j = 0; j < 3; j++) {
+            if (!labB[j]) continue;
+            for (int i = 0; i < 3; i++) {
+                const long long p = a.posTT[9 * b + 3 * j + i];
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&a.val[p * OO + bo * OD + ao], pb.tLP[N * (i * OD + ao) + j * OD + bo]);
+            }
+        }
+    }
+}
+
+// One block of 256 threads per outer element; threads stride over the element's
+// neighbour-list segment.  SEG 0: front segment (interior pairs; close pairs are left to the
+// tensor-Gauss kernel).  SEG 1: back segment (cut pairs, or baryCenter pairs).
+template <int KER, int OD, int SEG, typename T>
+__global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
+    constexpr int OO = OD * OD, NB = 9 * OO;
+    extern __shared__ int lds[];
+    int* hk = lds;
+    int* hv = hk + a.HN;
+    __shared__ double red[4][NB];
+    __shared__ unsigned subtri;
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nF = a.cntF[aT];
+    const int n = (SEG == 0) ? nF : a.cntB[aT];
+    if (n == 0) return;
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    const bool useHash = a.HN > 0;
+    if (useHash) {
+        for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+        __syncthreads();
+        for (int j = 0; j < 3; j++) {
+            const long long r0 = a.rowptr[rA[j]];
+            const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+            for (int i = tid; i < len; i += 256) {
+                int s = lds_slot(hk, a.HN, a.col[r0 + i]);
+                if (s >= 0) hv[3 * s + j] = i;
+                else atomicOr(&a.counters[CNT_ERROR], 16ull);
+            }
+        }
+        __syncthreads();
+    }
+    double tL[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) tL[i] = 0;
+    unsigned ntri = 0;
+    if (tid == 0) subtri = 0;
+    for (int p = tid; p < n; p += 256) {
+        const uint32_t e = a.nbr[(size_t)aT * a.cap + (SEG == 0 ? p : a.cap - 1 - p)];
+        const uint32_t cls = (e >> CLS_SHIFT) & 3u;
+        const int b = (int)(e & IDX_MASK);
+        if (SEG == 0 && cls == CLS_CLOSE) continue;
+        const ElemRec B = a.elem[b];
+        PairBlocks<OD> pb;
+        bool prime = true;
+        if constexpr (SEG == 0) {
+            pair_interior<KER, OD, T>(a, A, B, tL, pb);
+        } else {
+            if (e >> 31) {
+                pair_bary<KER, OD, T>(a, A, B, tL, pb);
+                prime = false;
+            } else {
+                pair_cut<KER, OD, T>(a, A, B, tL, pb, ntri);
+            }
+        }
+        scatter_pair<OD>(a, rA, labA, b, prime, pb, hk, hv, useHash);
+    }
+    // block reduction of termLocal, then +termLocal into (aAdx[i], aAdx[j])
+    const int lane = tid & 63, w = tid >> 6;
+    if (SEG == 1) atomicAdd(&subtri, ntri);
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        double v = tL[i];
+        for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+        if (lane == 0) red[w][i] = v;
+    }
+    __syncthreads();
+    constexpr int N = 3 * OD;
+    if (SEG == 1 && tid == 0) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)subtri);
+    for (int t = tid; t < NB; t += 256) {
+        const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        const int ra = t / N, cb = t % N;  // termLocal[N*a + b]
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (!labA[i]) continue;
+        const long long p = a.posTT[9 * aT + 3 * i + j];
+        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
+}
+
+// ======================================================================= close pairs
+struct CloseArgs {
+    int is_DG, nPg, od;
+    const ElemRec* elem;
+    const int* tri;
+    const int* vlabel;
+    const int2* list;
+    const double* tg;  // Duffy tables, 10 doubles per (case, traffo, point)
+    const double* dg;
+    const long long* rowptr;
+    const int* col;
+    double* val;
+    KernelParams kp;
+    unsigned long long* counters;
+};
+
+// Tensor-Gauss quadrature of a first-layer pair with a singular kernel: integrate_fractional
+// (src/integration.cpp:187-288) or integrate_weakSingular (:82-185) with join (:1913-1958).
+// One wavefront per pair; lanes stride over (traffo, point); wave reduction; lanes scatter.
+template <int KER, int OD, bool WEAK>
+__global__ __launch_bounds__(64) void k_close(CloseArgs a) {
+    constexpr int OO = OD * OD, N = 3 * OD, NB = 9 * OO;
+    constexpr int NACC = WEAK ? 3 * NB : 25 * OO;
+    __shared__ double blk[4][NB];
+    __shared__ double accs[NACC];
+    const int lane = threadIdx.x;
+    const int2 pr = a.list[blockIdx.x];
+    const int aT = pr.x, bT = pr.y;
+    // join: shared vertices first, in the order found (src/integration.cpp:1913-1958)
+    int aV[3], bV[3], aS[3], bS[3], argA[3], argB[3], AinB[3] = {0, 0, 0}, BinA[3] = {0, 0, 0};
+    for (int j = 0; j < 3; j++) { aV[j] = a.tri[3 * aT + j]; bV[j] = a.tri[3 * bT + j]; }
+    int nEqual = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (aV[i] == bV[j]) {
+                AinB[i]++; BinA[j]++;
+                aS[nEqual] = aV[i]; argA[nEqual] = i;
+                bS[nEqual] = bV[j]; argB[nEqual] = j;
+                nEqual++;
+            }
+    if (nEqual < 1 || nEqual > 3) {
+        if (lane == 0) atomicOr(&a.counters[CNT_ERROR], 32ull);
+        return;
+    }
+    {
+        int ia = 0, ib = 0;
+        for (int i = 0; i < 3; i++) {
+            if (!AinB[i]) { aS[nEqual + ia] = aV[i]; argA[nEqual + ia] = i; ia++; }
+            if (!BinA[i]) { bS[nEqual + ib] = bV[i]; argB[nEqual + ib] = i; ib++; }
+        }
+    }
+    const ElemRec EA = a.elem[aT], EB = a.elem[bT];
+    double as[6], bs[6];
+    for (int j = 0; j < 3; j++) {
+        int ja = 0, jb = 0;
+        for (int q = 0; q < 3; q++) { if (aV[q] == aS[j]) ja = q; if (bV[q] == bS[j]) jb = q; }
+        as[2 * j] = EA.E[2 * ja]; as[2 * j + 1] = EA.E[2 * ja + 1];
+        bs[2 * j] = EB.E[2 * jb]; bs[2 * j + 1] = EB.E[2 * jb + 1];
+    }
+    const double absA = absDet(as), absB = absDet(bs);
+    const int cs = nEqual - 1;
+    const int nT = cs == 0 ? 2 : (cs == 1 ? 5 : 6);
+    const long long base = (cs == 0 ? 0 : (cs == 1 ? 2 : 7)) * (long long)a.nPg * 10;
+    const int nLocal = 6 - nEqual, shift = 3 - nEqual;
+    double acc[NACC];
+#pragma unroll
+    for (int i = 0; i < NACC; i++) acc[i] = 0;
+    for (int it = lane; it < nT * a.nPg; it += 64) {
+        const int k = it % a.nPg;
+        const double* t = a.tg + base + (long long)it * 10;
+        const double det = WEAK ? t[9] : t[8];
+        double xc[2], yc[2];
+        toPhys(as, t[4], t[5], xc);
+        toPhys(bs, t[6], t[7], yc);
+        double K[OO];
+        kernel_eval<KER, OO, double>(xc[0] - yc[0], xc[1] - yc[1], a.kp, K);
+        const double wdg = det * a.dg[k];
+        if constexpr (WEAK) {
+            double px[3] = {1 - t[0] - t[1], t[0], t[1]};
+            double py[3] = {1 - t[2] - t[3], t[2], t[3]};
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+#pragma unroll
+                    for (int o = 0; o < OO; o++) {
+                        double f = K[o] * wdg;
+                        acc[(i * 3 + j) * OO + o] += f * px[i] * px[j];
+                        acc[NB + (i * 3 + j) * OO + o] += f * py[i] * py[j];
+                        acc[2 * NB + (i * 3 + j) * OO + o] += f * px[i] * py[j];
+                    }
+        } else {
+            double px[3] = {1 - t[4] - t[5], t[4], t[5]};
+            double py[3] = {1 - t[6] - t[7], t[6], t[7]};
+            double pxy[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < nEqual; i++) pxy[i] = px[i] - py[i];
+            for (int i = nEqual; i < 3; i++) { pxy[i] = px[i]; pxy[i + shift] = -py[i]; }
+#pragma unroll
+            for (int i = 0; i < 5; i++)
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+#pragma unroll
+                    for (int o = 0; o < OO; o++) acc[(i * 5 + j) * OO + o] += K[o] * wdg * pxy[i] * pxy[j];
+        }
+    }
+    // wave reduction into LDS
+#pragma unroll
+    for (int i = 0; i < NACC; i++) {
+        double v = acc[i];
+        for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+        if (lane == 0) accs[i] = v;
+    }
+    for (int i = lane; i < 4 * NB; i += 64) (&blk[0][0])[i] = 0;
+    __syncthreads();
+    const double ab = absA * absB;
+    if (lane == 0) {
+        double* tL = blk[0];
+        double* tN = blk[1];
+        double* tLP = blk[2];
+        double* tNP = blk[3];
+        if constexpr (WEAK) {
+            for (int i = 0; i < 3; i++)
+                for (int ao = 0; ao < OD; ao++)
+                    for (int j = 0; j < 3; j++)
+                        for (int bo = 0; bo < OD; bo++) {
+                            const int o = ao * OD + bo;
+                            tL[N * (argA[i] * OD + ao) + argA[j] * OD + bo] += accs[(i * 3 + j) * OO + o] * ab;
+                            tLP[N * (argB[i] * OD + ao) + argB[j] * OD + bo] += accs[NB + (i * 3 + j) * OO + o] * ab;
+                            tN[N * (argA[i] * OD + ao) + argB[j] * OD + bo] += accs[2 * NB + (i * 3 + j) * OO + o] * ab;
+                            tNP[N * (argA[i] * OD + ao) + argB[j] * OD + bo] += accs[2 * NB + (i * 3 + j) * OO + o] * ab;
+                        }
+        } else {
+            for (int i = 0; i < nLocal; i++)
+                for (int ao = 0; ao < OD; ao++)
+                    for (int j = 0; j < nLocal; j++)
+                        for (int bo = 0; bo < OD; bo++) {
+                            const double f = accs[(i * 5 + j) * OO + ao * OD + bo] * ab;
+                            if (i < 3) {
+                                if (j < 3) tL[N * (argA[i] * OD + ao) + argA[j] * OD + bo] += f;
+                                else tN[N * (argA[i] * OD + ao) + argB[j - shift] * OD + bo] -= f;
+                            } else {
+                                if (j < 3) tNP[N * (argA[j] * OD + ao) + argB[i - shift] * OD + bo] -= f;
+                                else tLP[N * (argB[i - shift] * OD + ao) + argB[j - shift] * OD + bo] += f;
+                            }
+                        }
+        }
+    }
+    __syncthreads();
+    // scatter (src/Cassemble.cpp:721-770); lanes split the 4*NB entries
+    int nA[3], nB[3];
+    for (int j = 0; j < 3; j++) { nA[j] = node_of(a.tri, aT, j, a.is_DG); nB[j] = node_of(a.tri, bT, j, a.is_DG); }
+    for (int t = lane; t < 4 * NB; t += 64) {
+        const int X = t / NB, e = t % NB, ra = e / N, cb = e % N;
+        const double v = blk[X][e];
+        if (v == 0.0) continue;
+        int rn, rc, cn, cc;
+        double sgn;
+        if (X == 0) { rn = nA[ra / OD]; rc = ra % OD; cn = nA[cb / OD]; cc = cb % OD; sgn = 1; }
+        else if (X == 1) { rn = nA[ra / OD]; rc = ra % OD; cn = nB[cb / OD]; cc = cb % OD; sgn = -1; }
+        else if (X == 2) { rn = nB[cb / OD]; rc = cb % OD; cn = nB[ra / OD]; cc = ra % OD; sgn = 1; }
+        else { rn = nB[cb / OD]; rc = cb % OD; cn = nA[ra / OD]; cc = ra % OD; sgn = -1; }
+        if (!row_labelled(a.vlabel, rn, a.is_DG)) continue;
+        const long long p = find_pos(a.rowptr, a.col, rn, cn);
+        if (p < 0) { atomicOr(&a.counters[CNT_ERROR], 64ull); continue; }
+        atomicAdd(&a.val[p * OO + rc * OD + cc], sgn * v);
+    }
+}
+
+// ======================================================================= compaction
+// One wavefront per node row: scalar rows od*r + o keep the nonzero entries, in column order.
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_compact(int nNodes, int od, const long long* rowptr, const int* col,
+                                                const double* val, int* ocnt, const long long* orow, int* ocol,
+                                                double* oval) {
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const long long r0 = rowptr[r], len = rowptr[r + 1] - r0;
+    const int oo = od * od;
+    for (int o = 0; o < od; o++) {
+        const long long total = len * od;
+        long long written = 0;
+        long long outBase = FILL ? orow[(long long)r * od + o] : 0;
+        for (long long t0 = 0; t0 < total; t0 += 64) {
+            const long long t = t0 + lane;
+            bool nz = false;
+            long long p = 0;
+            int oc = 0;
+            if (t < total) {
+                p = r0 + t / od;
+                oc = (int)(t % od);
+                nz = val[p * oo + o * od + oc] != 0.0;
+            }
+            const unsigned long long m = __ballot(nz);
+            if (FILL && nz) {
+                const long long idx = outBase + written + __popcll(m & ((1ull << lane) - 1ull));
+                ocol[idx] = col[p] * od + oc;
+                oval[idx] = val[p * oo + o * od + oc];
+            }
+            written += __popcll(m);
+        }
+        if (!FILL && lane == 0) ocnt[(long long)r * od + o] = (int)written;
+    }
+}
+
+// ======================================================================= host side
+static long long next_pow2(long long v) {
+    long long p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static void ev_begin(Plan& P, int ph) {
+    if (P.timing) HIPCHECK(hipEventRecord(P.ev[2 * ph], P.stream));
+}
+static void ev_end(Plan& P, int ph) {
+    if (P.timing) HIPCHECK(hipEventRecord(P.ev[2 * ph + 1], P.stream));
+}
+
+template <typename Tp>
+static void dalloc(Tp*& p, size_t n) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    if (n == 0) n = 1;
+    HIPCHECK(hipMalloc((void**)&p, n * sizeof(Tp)));
+}
+
+static void read_counters(Plan& P, unsigned long long* h) {
+    HIPCHECK(hipMemcpyAsync(h, P.d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+}
+
+// Duffy tables (src/MeshTypes.cpp:252-328, maps :32-209, scale/mirror
+// src/mathhelpers.cpp:88-97): per (case, traffo, point) 10 doubles
+// [alpha(4), alphaCanceled(4), detFractional, detWeak].
+static void duffy_map(int cs, int t, double* a, double* jac) {
+    const double xi = a[0], e1 = a[1], e2 = a[2], e3 = a[3];
+    double j;
+    if (cs == 0) {
+        if (t == 0) { a[0] = xi; a[1] = e1 * xi; a[2] = e2 * xi; a[3] = e2 * e3 * xi; }
+        else { a[0] = xi * e2; a[1] = xi * e2 * e3; a[2] = xi; a[3] = xi * e1; }
+        j = std::pow(xi, 3) * e2;
+    } else if (cs == 1) {
+        switch (t) {
+            case 0: a[0] = xi; a[1] = xi * e1 * e3; a[2] = xi * (1. - e1 * e2); a[3] = xi * e1 * (1. - e2); break;
+            case 1: a[0] = xi; a[1] = xi * e1; a[2] = xi * (1. - e1 * e2 * e3); a[3] = xi * e1 * e2 * (1. - e3); break;
+            case 2: a[0] = xi * (1. - e1 * e2); a[1] = xi * e1 * (1. - e2); a[2] = xi; a[3] = xi * e1 * e2 * e3; break;
+            case 3: a[0] = xi * (1. - e1 * e2 * e3); a[1] = xi * e1 * e2 * (1. - e3); a[2] = xi; a[3] = xi * e1; break;
+            default: a[0] = xi * (1. - e1 * e2 * e3); a[1] = xi * e1 * (1. - e2 * e3); a[2] = xi; a[3] = xi * e1 * e2; break;
+        }
+        j = (t == 0) ? std::pow(xi, 3) * (e1 * e1) : std::pow(xi, 3) * (e1 * e1) * e2;
+    } else {
+        switch (t) {
+            case 0: a[0] = xi; a[1] = xi * (1. - e1 + e1 * e2); a[2] = xi * (1. - e1 * e2 * e3); a[3] = xi * (1. - e1); break;
+            case 1: a[0] = xi * (1. - e1 * e2 * e3); a[1] = xi * (1. - e1); a[2] = xi; a[3] = xi * (1. - e1 + e1 * e2); break;
+            case 2: a[0] = xi; a[1] = xi * e1 * (1. - e2 + e2 * e3); a[2] = xi * (1. - e1 * e2); a[3] = xi * e1 * (1. - e2); break;
+            case 3: a[0] = xi * (1. - e1 * e2); a[1] = xi * e1 * (1. - e2); a[2] = xi; a[3] = xi * e1 * (1. - e2 + e2 * e3); break;
+            case 4: a[0] = xi * (1. - e1 * e2 * e3); a[1] = xi * e1 * (1. - e2 * e3); a[2] = xi; a[3] = xi * e1 * (1. - e2); break;
+            default: a[0] = xi; a[1] = xi * e1 * (1. - e2); a[2] = xi * (1. - e1 * e2 * e3); a[3] = xi * e1 * (1. - e2 * e3); break;
+        }
+        j = std::pow(xi, 3) * (e1 * e1) * e2;
+    }
+    *jac = j;
+}
+
+std::vector<double> duffy_tables(const std::vector<double>& Pg, int nPg, double s) {
+    std::vector<double> out((size_t)13 * nPg * 10);
+    size_t o = 0;
+    for (int cs = 0; cs < 3; cs++) {
+        const int nT = cs == 0 ? 2 : (cs == 1 ? 5 : 6);
+        for (int t = 0; t < nT; t++)
+            for (int k = 0; k < nPg; k++) {
+                double al[4], ac[4], jac;
+                for (int j = 0; j < 4; j++) { al[j] = Pg[4 * k + j] * 0.5 + 0.5; ac[j] = al[j]; }
+                double dw = std::pow(ac[0], 3 - 2 - 2 * s);
+                double df = std::pow(ac[0], 3 - 2 * s);
+                ac[0] = 1.0;
+                duffy_map(cs, t, al, &jac);
+                duffy_map(cs, t, ac, &jac);
+                dw *= jac * 0.0625;
+                df *= jac * 0.0625;
+                al[0] = al[0] - al[1]; al[2] = al[2] - al[3];
+                ac[0] = ac[0] - ac[1]; ac[2] = ac[2] - ac[3];
+                double* r = &out[o];
+                for (int j = 0; j < 4; j++) { r[j] = al[j]; r[4 + j] = ac[j]; }
+                r[8] = df;
+                r[9] = dw;
+                o += 10;
+            }
+    }
+    return out;
+}
+
+KernelParams kernel_params(const PlanConfig& c) {
+    KernelParams kp;
+    kp.sqdelta = c.sqdelta;
+    const double delta = std::sqrt(c.sqdelta);
+    kp.exponent = 0;
+    switch (c.kernel) {
+        case KER_CONSTANT: kp.c = 4 / (M_PI * (c.sqdelta * c.sqdelta)); break;               // src/model.cpp:25-28
+        case KER_PARABOLA: kp.c = 12. / (M_PI * std::pow(c.sqdelta, 3)); break;              // :52-60
+        case KER_FRACTIONAL:                                                                 // :126-138
+            kp.exponent = 2 + 2. * c.fractional_s;
+            kp.c = (2. - 2. * c.fractional_s) / (M_PI * std::pow(delta, 2. - 2. * c.fractional_s));
+            break;
+        case KER_MICRO: kp.c = 3.0 / (M_PI * std::pow(delta, 3)); break;                     // :116-124
+        default: kp.c = 3.0 / std::pow(delta, 3); break;                                     // :140-153
+    }
+    return kp;
+}
+
+// Upload the mesh and build the element records.
+void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* verts, const long* vlab64) {
+    const PlanConfig& c = P.cfg;
+    P.oo = c.outdim * c.outdim;
+    P.nNodes = c.is_DG ? 3 * c.nE : c.nV;
+    HIPCHECK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2 * PH_COUNT; i++) HIPCHECK(hipEventCreate(&P.ev[i]));
+    long* d_tri64;
+    long* d_lab64;
+    long* d_vlab64;
+    double* d_verts;
+    HIPCHECK(hipMalloc(&d_tri64, sizeof(long) * 3 * (size_t)c.nE));
+    HIPCHECK(hipMalloc(&d_lab64, sizeof(long) * (size_t)c.nE));
+    HIPCHECK(hipMalloc(&d_vlab64, sizeof(long) * (size_t)c.nV));
+    HIPCHECK(hipMalloc(&d_verts, sizeof(double) * 2 * (size_t)c.nV));
+    HIPCHECK(hipMemcpyAsync(d_tri64, tri64, sizeof(long) * 3 * (size_t)c.nE, hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipMemcpyAsync(d_lab64, lab64, sizeof(long) * (size_t)c.nE, hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipMemcpyAsync(d_vlab64, vlab64, sizeof(long) * (size_t)c.nV, hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipMemcpyAsync(d_verts, verts, sizeof(double) * 2 * (size_t)c.nV, hipMemcpyHostToDevice, P.stream));
+    dalloc(P.d_tri, 3 * (size_t)c.nE);
+    dalloc(P.d_elem, (size_t)c.nE);
+    dalloc(P.d_vlabel, (size_t)c.nV);
+    dalloc(P.d_counters, 16);
+    HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), P.stream));
+    k_setup<<<nblk(c.nE, 256), 256, 0, P.stream>>>(c.nE, d_tri64, d_lab64, d_verts, P.d_tri, P.d_elem,
+                                                   P.d_counters, c.nV);
+    k_vlabel<<<nblk(c.nV, 256), 256, 0, P.stream>>>(c.nV, d_vlab64, P.d_vlabel);
+    HIPCHECK(hipGetLastError());
+    if (c.nPg > 0) {
+        std::vector<double> tg = duffy_tables(c.Pg, c.nPg, c.fractional_s);
+        dalloc(P.d_tg, tg.size());
+        dalloc(P.d_dg, (size_t)c.nPg);
+        HIPCHECK(hipMemcpyAsync(P.d_tg, tg.data(), sizeof(double) * tg.size(), hipMemcpyHostToDevice, P.stream));
+        HIPCHECK(hipMemcpyAsync(P.d_dg, c.dg.data(), sizeof(double) * c.nPg, hipMemcpyHostToDevice, P.stream));
+    }
+    const size_t scan_n = 4u << 20;
+    HIPCHECK(hipMalloc(&P.d_scan_tmp, scan_n * sizeof(long long)));
+    P.scan_tmp_bytes = scan_n * sizeof(long long);
+    HIPCHECK(hipStreamSynchronize(P.stream));
+    unsigned long long h[16];
+    read_counters(P, h);
+    (void)hipFree(d_tri64);
+    (void)hipFree(d_lab64);
+    (void)hipFree(d_vlab64);
+    (void)hipFree(d_verts);
+    if (h[CNT_ERROR]) throw std::runtime_error("element references a vertex index out of range");
+}
+
+static long long* scan_scratch(Plan& P) { return (long long*)P.d_scan_tmp; }
+
+void plan_assemble(Plan& P, const Estimates& est) {
+    const PlanConfig& c = P.cfg;
+    const hipStream_t st = P.stream;
+    unsigned long long h[16];
+    HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+
+    // ---------------------------------------------------------------- dual graph
+    ev_begin(P, PH_DUAL);
+    int* d_cnt;
+    dalloc(d_cnt, (size_t)std::max(c.nE, c.nV) + 1);
+    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nV, st));
+    dalloc(P.d_vptr, (size_t)c.nV + 1);
+    dalloc(P.d_vel, 3 * (size_t)c.nE);
+    k_vinc_count<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, d_cnt);
+    scan_exclusive<int>(d_cnt, P.d_vptr, c.nV, scan_scratch(P), st);
+    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nV, st));
+    k_vinc_fill<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, P.d_vptr, d_cnt, P.d_vel);
+    k_sort_vinc<<<nblk(c.nV, 256), 256, 0, st>>>(c.nV, P.d_vptr, P.d_vel);
+    dalloc(P.d_xadj, (size_t)c.nE + 1);
+    k_dual<false><<<nblk(c.nE, 128), 128, 0, st>>>(c.nE, P.d_tri, P.d_vptr, P.d_vel, d_cnt, nullptr, nullptr,
+                                                   P.d_counters);
+    scan_exclusive<int>(d_cnt, P.d_xadj, c.nE, scan_scratch(P), st);
+    long long nadj = 0;
+    HIPCHECK(hipMemcpyAsync(&nadj, P.d_xadj + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    dalloc(P.d_adj, (size_t)nadj);
+    k_dual<true><<<nblk(c.nE, 128), 128, 0, st>>>(c.nE, P.d_tri, P.d_vptr, P.d_vel, nullptr, P.d_xadj, P.d_adj,
+                                                  P.d_counters);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_DUAL);
+
+    // ---------------------------------------------------------------- BFS
+    const double delta = std::sqrt(c.sqdelta);
+    double area_est = M_PI * std::pow(delta + 2.0 * est.hmax, 2) / est.amin;
+    long long visit = c.fullCC ? (long long)c.nE : (long long)(1.3 * area_est) + 128;
+    if (visit > c.nE) visit = c.nE;
+    dalloc(P.d_cntF, (size_t)c.nE);
+    dalloc(P.d_cntB, (size_t)c.nE);
+    const long long closeCap = (long long)c.nE * 32 + 1;
+    dalloc(P.d_closeList, (size_t)closeCap);
+    BfsArgs ba;
+    ba.nE = c.nE; ba.elem = P.d_elem; ba.xadj = P.d_xadj; ba.adj = P.d_adj; ba.q = c.q;
+    ba.sqdelta = c.sqdelta;
+    ba.sq_in = c.sqdelta * (1.0 - 1e-9);
+    ba.caps = c.caps;
+    ba.remote = c.method_remote;
+    ba.level0 = c.singular ? (c.method_close >= 2 ? 2 : c.method_close) : c.method_remote;
+    ba.fullCC = c.fullCC;
+    ba.cntF = nullptr;
+    ba.closeCap = closeCap;
+    for (int attempt = 0;; attempt++) {
+        long long cap = std::max<long long>(visit, 16);
+        long long H = next_pow2(2 * cap);
+        long long F = std::min<long long>(cap, 1 << 16);
+        size_t ldsb = sizeof(int) * (size_t)(H + 2 * F);
+        if (ldsb > 160 * 1024) {
+            // too large for LDS: fall back to the largest table that fits and rely on overflow detection
+            H = 16384;
+            F = std::min<long long>(F, 8192);
+            ldsb = sizeof(int) * (size_t)(H + 2 * F);
+        }
+        P.cap = (int)cap;
+        dalloc(P.d_nbr, (size_t)c.nE * (size_t)cap);
+        ba.cap = (int)cap; ba.H = (int)H; ba.F = (int)F;
+        ba.nbr = P.d_nbr; ba.cntF = P.d_cntF; ba.cntB = P.d_cntB; ba.closeList = P.d_closeList;
+        ba.counters = P.d_counters;
+        HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+        ev_begin(P, PH_BFS);
+        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+        k_bfs<<<c.nE, 64, ldsb, st>>>(ba);
+        HIPCHECK(hipGetLastError());
+        ev_end(P, PH_BFS);
+        read_counters(P, h);
+        if (h[CNT_ERROR]) throw std::runtime_error("mesh error (dual graph row longer than 128 elements)");
+        if (!h[CNT_OVERFLOW]) break;
+        if (attempt >= 3 || (H >= 16384 && ldsb > 64 * 1024 && visit >= c.nE))
+            throw std::length_error("BFS capacity overflow: interaction horizon too large for this build");
+        visit = std::min<long long>(2 * visit, c.nE);
+    }
+    P.stats.pairs = h[CNT_PAIRS];
+    P.stats.probes = h[CNT_PROBES];
+    P.stats.close = h[CNT_CLOSE];
+    P.stats.interior = h[CNT_INTERIOR];
+    P.stats.cut = h[CNT_CUT];
+    P.stats.bary = h[CNT_BARY];
+    P.stats.max_list = (int)h[CNT_MAXLIST];
+    P.stats.cap = P.cap;
+    const long long nclose = (long long)h[CNT_CLOSE];
+    if (nclose > closeCap) throw std::length_error("close pair list overflow");
+
+    // ---------------------------------------------------------------- transpose
+    ev_begin(P, PH_TRANSPOSE);
+    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
+    k_tcount<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt);
+    dalloc(P.d_tptr, (size_t)c.nE + 1);
+    scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
+    long long ntadj = 0;
+    HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    dalloc(P.d_tadj, (size_t)ntadj);
+    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
+    k_tfill<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, d_cnt, P.d_tadj);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_TRANSPOSE);
+
+    // ---------------------------------------------------------------- pattern
+    ev_begin(P, PH_PATTERN);
+    const double pat_est = M_PI * std::pow(delta + 3.0 * est.hmax, 2) / est.amin;
+    long long ecap = c.fullCC ? c.nE : std::min<long long>((long long)(1.4 * pat_est) + 256, c.nE);
+    PatArgs pa;
+    pa.nNodes = P.nNodes; pa.is_DG = c.is_DG; pa.cap = P.cap; pa.tri = P.d_tri; pa.vptr = P.d_vptr;
+    pa.vel = P.d_vel; pa.nbr = P.d_nbr; pa.cntF = P.d_cntF; pa.cntB = P.d_cntB; pa.tptr = P.d_tptr;
+    pa.tadj = P.d_tadj; pa.counters = P.d_counters;
+    int* d_rowcnt;
+    dalloc(d_rowcnt, (size_t)P.nNodes + 1);
+    dalloc(P.d_rowptr, (size_t)P.nNodes + 1);
+    for (int attempt = 0;; attempt++) {
+        pa.EH = (int)next_pow2(2 * ecap);
+        pa.DH = (int)next_pow2(c.is_DG ? 4 * ecap : ecap);
+        pa.SB = 0;
+        pa.rowcnt = d_rowcnt;
+        pa.rowptr = nullptr;
+        pa.col = nullptr;
+        size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH);
+        if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
+        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+        HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+        k_pattern<false><<<P.nNodes, 256, ldsb, st>>>(pa);
+        HIPCHECK(hipGetLastError());
+        read_counters(P, h);
+        if (!h[CNT_OVERFLOW]) break;
+        if (attempt >= 3) throw std::length_error("pattern capacity overflow");
+        ecap *= 2;
+    }
+    P.maxRow = (int)h[CNT_MAXROW];
+    scan_exclusive<int>(d_rowcnt, P.d_rowptr, P.nNodes, scan_scratch(P), st);
+    HIPCHECK(hipMemcpyAsync(&P.nnz_nodes, P.d_rowptr + P.nNodes, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (P.nnz_nodes >= (1ll << 31)) throw std::length_error("pattern exceeds 2^31 node entries");
+    dalloc(P.d_col, (size_t)P.nnz_nodes);
+    pa.SB = (int)next_pow2(std::max(P.maxRow, 1));
+    pa.rowptr = P.d_rowptr;
+    pa.col = P.d_col;
+    {
+        size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH + pa.SB);
+        if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
+        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+        k_pattern<true><<<P.nNodes, 256, ldsb, st>>>(pa);
+        HIPCHECK(hipGetLastError());
+    }
+    ev_end(P, PH_PATTERN);
+
+    // ---------------------------------------------------------------- positions
+    ev_begin(P, PH_POSITIONS);
+    int* d_rowOf;
+    dalloc(d_rowOf, (size_t)P.nnz_nodes);
+    dalloc(P.d_tpos, (size_t)P.nnz_nodes);
+    dalloc(P.d_posTT, 9 * (size_t)c.nE);
+    k_row_of<<<P.nNodes, 256, 0, st>>>(P.nNodes, P.d_rowptr, d_rowOf);
+    k_tpos<<<nblk(P.nnz_nodes, 256), 256, 0, st>>>(P.nnz_nodes, P.d_rowptr, P.d_col, d_rowOf, P.d_tpos, P.d_counters);
+    k_posTT<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, c.is_DG, P.d_rowptr, P.d_col, P.d_posTT, P.d_counters);
+    dalloc(P.d_val, (size_t)P.nnz_nodes * P.oo);
+    HIPCHECK(hipMemsetAsync(P.d_val, 0, sizeof(double) * (size_t)P.nnz_nodes * P.oo, st));
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_POSITIONS);
+    (void)hipFree(d_rowOf);
+
+    // ---------------------------------------------------------------- quadrature
+    NumArgs na;
+    na.nE = c.nE; na.cap = P.cap; na.is_DG = c.is_DG; na.caps = c.caps; na.od = c.outdim;
+    na.elem = P.d_elem; na.tri = P.d_tri; na.vlabel = P.d_vlabel; na.nbr = P.d_nbr; na.cntF = P.d_cntF;
+    na.cntB = P.d_cntB; na.rowptr = P.d_rowptr; na.col = P.d_col; na.tpos = P.d_tpos; na.posTT = P.d_posTT;
+    na.val = P.d_val; na.q = c.q; na.kp = kernel_params(c); na.counters = P.d_counters;
+    // LDS map of the three rows of the outer element: 16 bytes per slot
+    long long HN = next_pow2((long long)(2.5 * std::max(P.maxRow, 1)));
+    na.HN = (HN * 16 <= 64 * 1024) ? (int)HN : 0;
+    const size_t nlds = na.HN > 0 ? (size_t)na.HN * 16 : 0;
+
+    if (nclose > 0) {
+        ev_begin(P, PH_CLOSE);
+        CloseArgs ca;
+        ca.is_DG = c.is_DG; ca.nPg = c.nPg; ca.od = c.outdim; ca.elem = P.d_elem; ca.tri = P.d_tri;
+        ca.vlabel = P.d_vlabel; ca.list = P.d_closeList; ca.tg = P.d_tg; ca.dg = P.d_dg; ca.rowptr = P.d_rowptr;
+        ca.col = P.d_col; ca.val = P.d_val; ca.kp = na.kp; ca.counters = P.d_counters;
+        const bool weak = c.method_close == M_WEAKSINGULAR;
+#define LAUNCH_CLOSE(KER, OD)                                                                     \
+    do {                                                                                          \
+        if (weak) k_close<KER, OD, true><<<(unsigned)nclose, 64, 0, st>>>(ca);                     \
+        else k_close<KER, OD, false><<<(unsigned)nclose, 64, 0, st>>>(ca);                         \
+    } while (0)
+        switch (c.kernel) {
+            case KER_CONSTANT: LAUNCH_CLOSE(KER_CONSTANT, 1); break;
+            case KER_PARABOLA: LAUNCH_CLOSE(KER_PARABOLA, 1); break;
+            case KER_FRACTIONAL: LAUNCH_CLOSE(KER_FRACTIONAL, 1); break;
+            case KER_MICRO: LAUNCH_CLOSE(KER_MICRO, 1); break;
+            default: LAUNCH_CLOSE(KER_MICRO_FIELD, 2); break;
+        }
+#undef LAUNCH_CLOSE
+        HIPCHECK(hipGetLastError());
+        ev_end(P, PH_CLOSE);
+    }
+
+#define LAUNCH_PAIRS(KER, OD, SEG)                                                                          \
+    do {                                                                                                    \
+        if (c.precision == 1) k_pairs<KER, OD, SEG, float><<<c.nE, 256, nlds, st>>>(na);                    \
+        else k_pairs<KER, OD, SEG, double><<<c.nE, 256, nlds, st>>>(na);                                    \
+    } while (0)
+#define DISPATCH_PAIRS(SEG)                                                        \
+    do {                                                                           \
+        switch (c.kernel) {                                                        \
+            case KER_CONSTANT: LAUNCH_PAIRS(KER_CONSTANT, 1, SEG); break;          \
+            case KER_PARABOLA: LAUNCH_PAIRS(KER_PARABOLA, 1, SEG); break;          \
+            case KER_FRACTIONAL: LAUNCH_PAIRS(KER_FRACTIONAL, 1, SEG); break;      \
+            case KER_MICRO: LAUNCH_PAIRS(KER_MICRO, 1, SEG); break;                \
+            default: LAUNCH_PAIRS(KER_MICRO_FIELD, 2, SEG); break;                 \
+        }                                                                          \
+    } while (0)
+    ev_begin(P, PH_INTERIOR);
+    DISPATCH_PAIRS(0);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_INTERIOR);
+    ev_begin(P, PH_CUT);
+    DISPATCH_PAIRS(1);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_CUT);
+#undef DISPATCH_PAIRS
+#undef LAUNCH_PAIRS
+
+    // ---------------------------------------------------------------- compaction
+    ev_begin(P, PH_COMPACT);
+    const long long K = c.K;
+    int* d_ocnt;
+    dalloc(d_ocnt, (size_t)K + 1);
+    k_compact<false><<<P.nNodes, 64, 0, st>>>(P.nNodes, c.outdim, P.d_rowptr, P.d_col, P.d_val, d_ocnt, nullptr,
+                                              nullptr, nullptr);
+    dalloc(P.d_orow, (size_t)K + 1);
+    scan_exclusive<int>(d_ocnt, P.d_orow, K, scan_scratch(P), st);
+    HIPCHECK(hipMemcpyAsync(&P.nnz, P.d_orow + K, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    dalloc(P.d_ocol, (size_t)P.nnz);
+    dalloc(P.d_oval, (size_t)P.nnz);
+    k_compact<true><<<P.nNodes, 64, 0, st>>>(P.nNodes, c.outdim, P.d_rowptr, P.d_col, P.d_val, nullptr, P.d_orow,
+                                             P.d_ocol, P.d_oval);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_COMPACT);
+    read_counters(P, h);
+    (void)hipFree(d_ocnt);
+    (void)hipFree(d_cnt);
+    (void)hipFree(d_rowcnt);
+    if (h[CNT_ERROR]) throw std::runtime_error("assembly consistency error (code " + std::to_string(h[CNT_ERROR]) + ")");
+    P.stats.nnz_nodes = P.nnz_nodes;
+    P.stats.nnz = P.nnz;
+    P.stats.subtri = h[CNT_SUBTRI];
+    if (P.timing) {
+        for (int ph = PH_DUAL; ph < PH_COUNT; ph++) {
+            float ms = 0;
+            if (ph == PH_CLOSE && nclose == 0) { P.stats.ms[ph] = 0; continue; }
+            HIPCHECK(hipEventElapsedTime(&ms, P.ev[2 * ph], P.ev[2 * ph + 1]));
+            P.stats.ms[ph] = ms;
+        }
+    }
+    P.assembled = true;
+}
+
+void plan_free(Plan& P) {
+    void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
+                    P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
+                    P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int i = 0; i < 2 * PH_COUNT; i++)
+        if (P.ev[i]) (void)hipEventDestroy(P.ev[i]);
+    if (P.stream) (void)hipStreamDestroy(P.stream);
+}
+
+}  // namespace nlfem

@@ -1,0 +1,103 @@
+// Host-side plan of one stiffness assembly on one GPU: configuration, device buffers,
+// phase timings.  The plan owns everything; all entry points take it by argument, so the
+// library is reentrant per plan (the reference keeps global function pointers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "nlfem_device.hpp"
+
+namespace nlfem {
+
+constexpr int MAXQ = 32;  // max points of the outer / inner triangle rules
+
+struct QuadArgs {
+    int nPx, nPy;
+    double Px[2 * MAXQ], dx[MAXQ], Py[2 * MAXQ], dy[MAXQ];
+    double psix[3 * MAXQ], psiy[3 * MAXQ];
+};
+
+struct PlanConfig {
+    int K = 0, nE = 0, nV = 0, outdim = 1, is_DG = 0;
+    int kernel = KER_CONSTANT, method_remote = M_RETRIANGULATE, method_close = M_RETRIANGULATE;
+    int caps = 1, singular = 0, fullCC = 0;
+    double sqdelta = 0, fractional_s = -1;
+    int degree = 0, nPg = 0;
+    int precision = 0;  // 0: FP64 throughout; 1: mixed (FP32 quadrature sums, FP64 accumulation)
+    QuadArgs q;
+    std::vector<double> Pg, dg;
+};
+
+// Phase timings in milliseconds (hipEvent pairs on the plan's stream).
+enum Phase : int {
+    PH_SETUP = 0, PH_DUAL, PH_BFS, PH_TRANSPOSE, PH_PATTERN, PH_POSITIONS, PH_CLOSE, PH_INTERIOR, PH_CUT,
+    PH_COMPACT, PH_COUNT
+};
+
+struct Stats {
+    unsigned long long pairs = 0, probes = 0, close = 0, interior = 0, cut = 0, bary = 0, subtri = 0;
+    long long nnz_nodes = 0, nnz = 0;
+    int cap = 0, max_list = 0;
+    float ms[PH_COUNT] = {0};
+};
+
+// Geometric size estimates for the LDS capacities (grown and retried on overflow).
+struct Estimates {
+    double hmax = 0, amin = 1e300;
+};
+
+struct Plan {
+    PlanConfig cfg;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2 * PH_COUNT] = {};
+    bool timing = true;
+    int oo = 1;
+    int nNodes = 0;  // CG: nV, DG: 3 nE
+
+    // mesh
+    int* d_tri = nullptr;        // [nE*3] vertex ids
+    int* d_vlabel = nullptr;     // [nV]
+    ElemRec* d_elem = nullptr;   // [nE]
+    // vertex -> element incidence
+    long long* d_vptr = nullptr; // [nV+1]
+    int* d_vel = nullptr;
+    // dual graph
+    long long* d_xadj = nullptr; // [nE+1]
+    int* d_adj = nullptr;
+    // neighbour lists (padded: cap slots per outer element; front = interior/close, back = cut/bary)
+    int cap = 0;
+    uint32_t* d_nbr = nullptr;
+    int* d_cntF = nullptr;
+    int* d_cntB = nullptr;
+    int2* d_closeList = nullptr;
+    unsigned long long* d_counters = nullptr;  // [16]: stats + close count + flags
+    // transpose of the neighbour relation
+    long long* d_tptr = nullptr;
+    int* d_tadj = nullptr;
+    // node CSR (pattern of the full symmetric matrix in node space, od x od blocks per entry)
+    long long* d_rowptr = nullptr;  // [nNodes+1]
+    int* d_col = nullptr;           // [nnz_nodes]
+    int* d_tpos = nullptr;          // [nnz_nodes] position of the transposed entry
+    int* d_posTT = nullptr;         // [nE*9] positions of (node(T,i), node(T,j))
+    double* d_val = nullptr;        // [nnz_nodes*oo]
+    long long nnz_nodes = 0;
+    int maxRow = 0;
+    // Duffy tables [case][traffo][k]: alpha (4), alphaCanceled (4), det (1)
+    double* d_tg = nullptr;
+    double* d_dg = nullptr;
+    // final scalar CSR
+    long long* d_orow = nullptr;  // [K+1]
+    int* d_ocol = nullptr;
+    double* d_oval = nullptr;
+    long long nnz = 0;
+    // scratch
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+
+    Stats stats;
+    bool assembled = false;
+};
+
+}  // namespace nlfem

@@ -1,0 +1,75 @@
+"""Shared test helpers: reference configurations and matrix comparison."""
+import numpy as np
+import scipy.sparse as sp
+
+from nlfem_amd.config import make_configuration
+from nlfem_amd.mesh import unit_square
+
+
+def case(name):
+    """(mesh, kernel, configuration) for the parity cases (small versions of the benchmark configs)."""
+    if name == "C1":  # h=1/16, delta=0.1, constant, retriangulate + caps, 7/7
+        return unit_square(16, 0.1), {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration()
+    if name == "C2s":  # parabola, jittered +-0.2h
+        return (unit_square(24, 0.1, jitter=0.2, seed=7), {"function": "parabola", "horizon": 0.1, "outputdim": 1},
+                make_configuration())
+    if name == "C3s":  # fractional s=0.4, tensor Gauss close pairs
+        return (unit_square(12, 0.2), {"function": "fractional", "horizon": 0.2, "outputdim": 1, "fractional_s": 0.4},
+                make_configuration(close="fractional", tensorGaussDegree=3))
+    if name == "C5s":  # microelastic field, outdim 2, 16-pt outer / 7-pt inner, close retriangulate
+        return (unit_square(12, 0.15, jitter=0.2, seed=3, outdim=2),
+                {"function": "linearPrototypeMicroelasticField", "horizon": 0.15, "outputdim": 2, "fractional_s": -0.5},
+                make_configuration(close="retriangulate", outer="16point", inner="7point"))
+    if name == "DG":
+        return (unit_square(10, 0.2, ansatz="DG"), {"function": "constant", "horizon": 0.2, "outputdim": 1},
+                make_configuration(ansatz="DG"))
+    if name == "bary":
+        return (unit_square(16, 0.15, jitter=0.1, seed=1), {"function": "parabola", "horizon": 0.15, "outputdim": 1},
+                make_configuration(method="baryCenter"))
+    if name == "weak":
+        return (unit_square(10, 0.2), {"function": "linearPrototypeMicroelastic", "horizon": 0.2, "outputdim": 1,
+                                       "fractional_s": -0.5},
+                make_configuration(close="weakSingular", tensorGaussDegree=3))
+    if name == "nocaps":
+        return (unit_square(14, 0.12, jitter=0.25, seed=11), {"function": "constant", "horizon": 0.12, "outputdim": 1},
+                make_configuration(caps=0))
+    raise KeyError(name)
+
+
+def label0_mesh():
+    """Unit-square mesh whose middle column of squares is label 0 (traversed, never integrated)."""
+    m, k, c = case("C1")
+    n = 16
+    el = m["elements"]
+    cx = m["vertices"][el].mean(axis=1)[:, 0]
+    lab = m["elementLabels"].copy()
+    lab[(cx > 0.5) & (cx < 0.5 + 1.0 / n)] = 0
+    m = dict(m)
+    m["elementLabels"] = lab
+    m["nE_Omega"] = int(np.sum(lab > 0))
+    return m, k, c
+
+
+def compare(A, B, rtol=1e-12):
+    """Entrywise comparison of two CSR matrices with identical patterns.
+
+    Returns (pattern_equal, max relative error, max error relative to the row scale)."""
+    A = sp.csr_matrix(A)
+    B = sp.csr_matrix(B)
+    A.sort_indices()
+    B.sort_indices()
+    same = A.shape == B.shape and np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+    D = (A - B).tocsr()
+    if D.nnz == 0:
+        return same, 0.0, 0.0
+    absB = abs(B).tocsr()
+    rowmax = np.maximum(np.asarray(absB.max(axis=1).todense()).ravel(), 1e-300)
+    Dc = D.tocoo()
+    bvals = np.asarray(abs(B)[Dc.row, Dc.col]).ravel()
+    rel = np.abs(Dc.data) / np.maximum(bvals, 1e-300)
+    rowrel = np.abs(Dc.data) / rowmax[Dc.row]
+    return same, float(rel.max()), float(rowrel.max())
+
+
+def sorted_lists(ptr, nbr):
+    return [np.sort(nbr[ptr[i]:ptr[i + 1]]) for i in range(len(ptr) - 1)]

@@ -1,0 +1,161 @@
+"""GPU parity: libnlfem_amd.so (HIP, through the C ABI) against the FP64 CPU oracle.
+
+Bar: neighbour lists and sparsity pattern bit-exact; FP64 entries within 1e-12 of the
+oracle relative to the entry (or, for entries that cancel, 1e-12 of their row's largest
+entry); mixed precision within 1e-5 relative Frobenius error.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from helpers import case, compare, label0_mesh, sorted_lists
+from nlfem_amd import nlfem
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _gpu(mesh, kernel, conf):
+    asm = nlfem.Assembler(mesh, kernel, conf)
+    try:
+        asm.assemble()
+        A = asm.csr()
+        nb = asm.neighbours()
+        st = asm.stats()
+    finally:
+        asm.close()
+    return A, nb, st
+
+
+def _check_case(name, mesh, kernel, conf, tol=FP64_TOL):
+    print(f"[parity] case={name} nE={mesh['nE']} kernel={kernel['function']}", flush=True)
+    ref = oracle.par_system(mesh, kernel, conf, keep_neighbours=True)
+    A, (ptr, nbr, cls), st = _gpu(mesh, kernel, conf)
+    # neighbour lists: identical sets per outer element
+    assert np.array_equal(ptr, ref["nbr_ptr"]), "neighbour list lengths differ"
+    for g, r in zip(sorted_lists(ptr, nbr), sorted_lists(ref["nbr_ptr"], ref["nbr"])):
+        assert np.array_equal(g, r)
+    # close (first-layer tensor Gauss) pairs are flagged identically
+    if conf.get("closeElements") in ("fractional", "weakSingular"):
+        gclose = sorted((i, b) for i in range(len(ptr) - 1) for b, c in
+                        zip(nbr[ptr[i]:ptr[i + 1]], cls[ptr[i]:ptr[i + 1]]) if c == 2)
+        rclose = sorted((i, b) for i in range(len(ptr) - 1) for b, c in
+                        zip(ref["nbr"][ref["nbr_ptr"][i]:ref["nbr_ptr"][i + 1]],
+                            ref["nbr_close"][ref["nbr_ptr"][i]:ref["nbr_ptr"][i + 1]]) if c)
+        assert gclose == rclose
+    assert st["pairs"] == ref["n_pairs"] and st["probes"] == ref["n_probes"]
+    same, rel, rowrel = compare(A, ref["A"])
+    print(f"[parity] case={name} pairs={int(st['pairs'])} nnz={A.nnz} pattern_equal={same} "
+          f"max_rel={rel:.3e} max_rowrel={rowrel:.3e}", flush=True)
+    assert same, "sparsity pattern differs"
+    D = (A - ref["A"]).tocoo()
+    absR = abs(ref["A"]).tocsr()
+    rowmax = np.asarray(absR.max(axis=1).todense()).ravel()
+    bound = tol * np.maximum(np.abs(np.asarray(ref["A"][D.row, D.col]).ravel()), rowmax[D.row])
+    assert np.all(np.abs(D.data) <= bound), f"max rel {rel:.3e}"
+    return A, ref
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "C3s", "C5s", "DG", "bary", "weak", "nocaps"])
+def test_parity_cases(name):
+    mesh, kernel, conf = case(name)
+    _check_case(name, mesh, kernel, conf)
+
+
+def test_parity_label0_traversal():
+    mesh, kernel, conf = label0_mesh()
+    _check_case("label0", mesh, kernel, conf)
+
+
+def test_parity_full_connected_component():
+    from nlfem_amd.mesh import unit_square
+    from nlfem_amd.config import make_configuration
+    mesh = unit_square(6, 0.1)
+    conf = make_configuration()
+    conf["is_fullConnectedComponentSearch"] = 1
+    _check_case("fullCC", mesh, {"function": "constant", "horizon": 0.1, "outputdim": 1}, conf)
+
+
+def test_mixed_precision_tolerance():
+    mesh, kernel, conf = case("C2s")
+    conf = dict(conf)
+    conf["precision"] = "mixed"
+    ref = oracle.par_system(mesh, kernel, conf)
+    A, _, _ = _gpu(mesh, kernel, conf)
+    same, rel, rowrel = compare(A, ref["A"])
+    fro = sp.linalg.norm(A - ref["A"]) / sp.linalg.norm(ref["A"])
+    print(f"[parity] mixed C2s rel_fro={fro:.3e} max_rowrel={rowrel:.3e}", flush=True)
+    assert same
+    assert fro < 1e-5
+
+
+def test_retriangulate_batch_bitexact():
+    rng = np.random.default_rng(5)
+    n = 4000
+    TE = rng.uniform(0, 1, size=(n, 6))
+    x = rng.uniform(-0.2, 1.2, size=(n, 2))
+    sq = 0.3 ** 2
+    # tangent and vertex-on-circle cases
+    TE[:50] = [0, 0, 1, 0, 0, 1]
+    x[:50] = np.stack([np.linspace(0.1, 0.9, 50), np.full(50, -0.3)], axis=1)
+    TE[50:100] = [0, 0, 0.3, 0, 0, 0.5]
+    x[50:100] = [0, 0]
+    ntri, lists = nlfem.retriangulate_batch(x, TE, sq, caps=1)
+    for i in range(n):
+        r, l = oracle.retriangulate(x[i], TE[i], sq, caps=1)
+        assert ntri[i] == r, i
+        assert np.array_equal(lists[i, :6 * r], l[:6 * r]), i
+    ntri0, lists0 = nlfem.retriangulate_batch(x, TE, sq, caps=0)
+    for i in range(0, n, 7):
+        r, l = oracle.retriangulate(x[i], TE[i], sq, caps=0)
+        assert ntri0[i] == r and np.array_equal(lists0[i, :6 * r], l[:6 * r])
+
+
+def test_geometry_helpers():
+    E = np.array([0.1, 0.2, 0.9, 0.25, 0.3, 0.8])
+    p = np.array([0.2, 0.3])
+    x = nlfem.py_toPhys(E, p)
+    assert np.array_equal(x, oracle.to_phys(E, p))
+    assert np.array_equal(nlfem.py_toRef(E, x), oracle.to_ref(E, x))
+    A = np.array([0.5, 2.0, 3.0, -1.0])
+    b = np.array([1.0, 2.0])
+    assert np.array_equal(nlfem.py_solve2x2(A, b), oracle.solve2x2(A, b))
+    with pytest.raises(nlfem.NlfemError):
+        nlfem.py_solve2x2(np.zeros(4), b)
+
+
+def test_file_contract_matches_in_memory(tmp_path):
+    mesh, kernel, conf = case("C1")
+    A = nlfem.stiffnessMatrix(mesh, kernel, conf)
+    conf2 = dict(conf)
+    conf2["savePath"] = str(tmp_path / "A.arma")
+    B = nlfem.stiffnessMatrix(mesh, kernel, conf2)
+    with open(conf2["savePath"], "rb") as f:
+        assert f.readline() == b"ARMA_SPM_BIN_FN008\n"
+    assert (A != B).nnz == 0
+
+
+def test_load_vector_and_mass():
+    mesh, kernel, conf = case("C2s")
+    f = nlfem.loadVector(mesh, {"function": "linear"}, conf)
+    fr = oracle.load_vector(mesh, {"function": "linear"}, conf)
+    assert np.allclose(f, fr, rtol=1e-13, atol=1e-15)
+    u = np.random.default_rng(0).normal(size=mesh["nV"])
+    v = nlfem.evaluateMass(mesh, u, conf["quadrature"]["outer"]["points"], conf["quadrature"]["outer"]["weights"],
+                           J=mesh["nE"])
+    vr = oracle.evaluate_mass(mesh, u, conf["quadrature"]["outer"]["points"], conf["quadrature"]["outer"]["weights"])
+    assert np.allclose(v, vr, rtol=1e-12, atol=1e-15)
+
+
+def test_invalid_configuration_raises():
+    mesh, kernel, conf = case("C1")
+    with pytest.raises(nlfem.NlfemError):
+        nlfem.stiffnessMatrix(mesh, {"function": "nope", "horizon": 0.1, "outputdim": 1}, conf)
+    bad = dict(conf)
+    bad["approxBalls"] = {"method": "fractional"}
+    with pytest.raises(nlfem.NlfemError):
+        nlfem.stiffnessMatrix(mesh, kernel, bad)
