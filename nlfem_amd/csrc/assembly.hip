@@ -1278,7 +1278,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- dual graph
     ev_begin(P, PH_DUAL);
-    int* d_cnt;
+    int* d_cnt = nullptr;
     dalloc(d_cnt, (size_t)std::max(c.nE, c.nV) + 1);
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nV, st));
     dalloc(P.d_vptr, (size_t)c.nV + 1);
@@ -1383,7 +1383,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     pa.nNodes = P.nNodes; pa.is_DG = c.is_DG; pa.cap = P.cap; pa.tri = P.d_tri; pa.vptr = P.d_vptr;
     pa.vel = P.d_vel; pa.nbr = P.d_nbr; pa.cntF = P.d_cntF; pa.cntB = P.d_cntB; pa.tptr = P.d_tptr;
     pa.tadj = P.d_tadj; pa.counters = P.d_counters;
-    int* d_rowcnt;
+    int* d_rowcnt = nullptr;
     dalloc(d_rowcnt, (size_t)P.nNodes + 1);
     dalloc(P.d_rowptr, (size_t)P.nNodes + 1);
     for (int attempt = 0;; attempt++) {
@@ -1424,7 +1424,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- positions
     ev_begin(P, PH_POSITIONS);
-    int* d_rowOf;
+    int* d_rowOf = nullptr;
     dalloc(d_rowOf, (size_t)P.nnz_nodes);
     dalloc(P.d_tpos, (size_t)P.nnz_nodes);
     dalloc(P.d_posTT, 9 * (size_t)c.nE);
@@ -1501,7 +1501,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // ---------------------------------------------------------------- compaction
     ev_begin(P, PH_COMPACT);
     const long long K = c.K;
-    int* d_ocnt;
+    int* d_ocnt = nullptr;
     dalloc(d_ocnt, (size_t)K + 1);
     k_compact<false><<<P.nNodes, 64, 0, st>>>(P.nNodes, c.outdim, P.d_rowptr, P.d_col, P.d_val, d_ocnt, nullptr,
                                               nullptr, nullptr);
