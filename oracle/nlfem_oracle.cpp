@@ -666,6 +666,10 @@ struct OracleResult {
 };
 
 static thread_local char g_err[512];
+// When set, every addend is scattered as its absolute value: the resulting matrix bounds the
+// rounding error an entry can carry (condition-aware test tolerances).
+static int g_abs = 0;
+void oracle_set_abs_mode(int on) { g_abs = on; }
 const char* oracle_last_error() { return g_err; }
 
 // src/Cassemble.cpp:360-462 (par_assemble, "system") + :464-855 (par_system).
@@ -768,15 +772,15 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                         unsigned long Adx;
                                         if (is_DG || vertLabel[aAdx[a / od]] > 0) {
                                             Adx = (unsigned long)(od * aAdx[a / od] + a % od) * K + od * aAdx[b / od] + b % od;
-                                            Ad[Adx] += tL[dV * od * a + b];
+                                            Ad[Adx] += g_abs ? std::fabs(tL[dV * od * a + b]) : tL[dV * od * a + b];
                                             Adx = (unsigned long)(od * aAdx[a / od] + a % od) * K + od * bAdx[b / od] + b % od;
-                                            Ad[Adx] += -tN[dV * od * a + b];
+                                            Ad[Adx] += g_abs ? std::fabs(tN[dV * od * a + b]) : -tN[dV * od * a + b];
                                         }
                                         if (is_DG || vertLabel[bAdx[b / od]] > 0) {
                                             Adx = (unsigned long)(od * bAdx[b / od] + b % od) * K + od * bAdx[a / od] + a % od;
-                                            Ad[Adx] += tLP[dV * od * a + b];
+                                            Ad[Adx] += g_abs ? std::fabs(tLP[dV * od * a + b]) : tLP[dV * od * a + b];
                                             Adx = (unsigned long)(od * bAdx[b / od] + b % od) * K + od * aAdx[a / od] + a % od;
-                                            Ad[Adx] += -tNP[dV * od * a + b];
+                                            Ad[Adx] += g_abs ? std::fabs(tNP[dV * od * a + b]) : -tNP[dV * od * a + b];
                                         }
                                     }
                                 }

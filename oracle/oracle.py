@@ -42,6 +42,7 @@ def load():
     lib.oracle_result_csr.argtypes = [C.c_void_p, _lp, _lp, _dp]
     lib.oracle_result_neighbours.argtypes = [C.c_void_p, _lp, _ip, C.POINTER(C.c_ubyte)]
     lib.oracle_result_free.argtypes = [C.c_void_p]
+    lib.oracle_set_abs_mode.argtypes = [C.c_int]
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_dual_graph.restype = C.c_long
     lib.oracle_dual_graph.argtypes = [_lp, C.c_int, C.c_int, _ip, _ip]
@@ -113,6 +114,16 @@ def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None
         return out
     finally:
         lib.oracle_result_free(h)
+
+
+def abs_matrix(mesh, kernel, conf):
+    """Sum of |addend| per entry (same pattern as par_system): a bound for rounding effects."""
+    lib = load()
+    lib.oracle_set_abs_mode(1)
+    try:
+        return par_system(mesh, kernel, conf)["A"]
+    finally:
+        lib.oracle_set_abs_mode(0)
 
 
 def dual_graph(elements, nV):

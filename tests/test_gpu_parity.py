@@ -1,8 +1,10 @@
 """GPU parity: libnlfem_amd.so (HIP, through the C ABI) against the FP64 CPU oracle.
 
-Bar: neighbour lists and sparsity pattern bit-exact; FP64 entries within 1e-12 of the
-oracle relative to the entry (or, for entries that cancel, 1e-12 of their row's largest
-entry); mixed precision within 1e-5 relative Frobenius error.
+Bar: neighbour lists and sparsity pattern bit-exact; every FP64 entry within 1e-12 of the
+oracle, relative to the sum of the absolute values of the addends that formed it (for
+entries without cancellation that is the entry itself; the microelastic 16/7-point case has
+near-coincident outer and inner points on identical pairs, whose addends exceed the entries
+by ~1e6); mixed precision within 1e-5 relative Frobenius error.
 """
 import os
 
@@ -53,9 +55,10 @@ def _check_case(name, mesh, kernel, conf, tol=FP64_TOL):
           f"max_rel={rel:.3e} max_rowrel={rowrel:.3e}", flush=True)
     assert same, "sparsity pattern differs"
     D = (A - ref["A"]).tocoo()
-    absR = abs(ref["A"]).tocsr()
-    rowmax = np.asarray(absR.max(axis=1).todense()).ravel()
-    bound = tol * np.maximum(np.abs(np.asarray(ref["A"][D.row, D.col]).ravel()), rowmax[D.row])
+    Abs = oracle.abs_matrix(mesh, kernel, conf).tocsr()
+    bound = tol * np.asarray(Abs[D.row, D.col]).ravel()
+    worst = float(np.max(np.abs(D.data) / np.maximum(bound / tol, 1e-300))) if D.nnz else 0.0
+    print(f"[parity] case={name} max |gpu-oracle| / sum|addends| = {worst:.3e}", flush=True)
     assert np.all(np.abs(D.data) <= bound), f"max rel {rel:.3e}"
     return A, ref
 
@@ -136,7 +139,9 @@ def test_file_contract_matches_in_memory(tmp_path):
     B = nlfem.stiffnessMatrix(mesh, kernel, conf2)
     with open(conf2["savePath"], "rb") as f:
         assert f.readline() == b"ARMA_SPM_BIN_FN008\n"
-    assert (A != B).nnz == 0
+    # same pattern; values agree to rounding (FP64 atomics sum in a run-dependent order)
+    assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+    assert abs(A - B).max() <= 1e-14 * abs(A).max()
 
 
 def test_load_vector_and_mass():
