@@ -428,7 +428,7 @@ __global__ void k_tfill(int nE, int cap, const uint32_t* nbr, const int* cntF, c
         uint32_t e = nbr[(size_t)aT * cap + list_index(p, cntF[aT], cap)];
         int b = (int)(e & IDX_MASK);
         int s = atomicAdd(&tfill[b], 1);
-        tadj[tptr[b] + s] = aT;
+        tadj[tptr[b] + s] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
     }
 }
 
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
         for (int p = tid; p < n + nt; p += 256) {
             int U;
             if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
-            else U = a.tadj[tb + (p - n)];
+            else U = (int)((uint32_t)a.tadj[tb + (p - n)] & IDX_MASK);
             if (lds_insert(eh, a.EH, U) < 0) ovf = 1;
         }
     }
@@ -592,6 +592,11 @@ struct NumArgs {
     QuadArgs q;
     KernelParams kp;
     unsigned long long* counters;
+    // interior pairs, owner computes: inverse relation and per-(element, local vertex) partial rows
+    const long long* tptr;
+    const int* tadj;
+    const long long* poff;
+    double* P;
 };
 
 // Kernel value(s) at z = x - y: OO entries (row-major od x od).
@@ -625,7 +630,7 @@ struct PairBlocks {
 // Contribution of one inner quadrature point y (with weight w = dy_i * |det rT|, basis values
 // psi at y, kernel K = k(x_k, y)) to the inner sums of outer point k, and to termLocalPrime.
 // Follows src/integration.cpp:424-476.
-template <int OD, typename T>
+template <int OD, typename T, bool WL = true, bool WN = true, bool WLP = true>
 NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double wk, double* inL, double* inN, double* tLP) {
 #pragma clang fp contract(fast)
     constexpr int OO = OD * OD;
@@ -633,26 +638,30 @@ NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double wk, double* inL
 #pragma unroll
     for (int o = 0; o < OO; o++) {
         kw[o] = K[o] * w;
-        inL[o] += (double)kw[o];
+        if (WL) inL[o] += (double)kw[o];
     }
+    if (WN) {
 #pragma unroll
-    for (int b = 0; b < 3; b++)
+        for (int b = 0; b < 3; b++)
 #pragma unroll
-        for (int o = 0; o < OO; o++) inN[b * OO + o] += (double)(psi[b] * kw[o]);
+            for (int o = 0; o < OO; o++) inN[b * OO + o] += (double)(psi[b] * kw[o]);
+    }
+    if (WLP) {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) {
-            T pp = psi[a] * psi[b];
+            for (int b = 0; b < 3; b++) {
+                T pp = psi[a] * psi[b];
 #pragma unroll
-            for (int o = 0; o < OO; o++)
-                tLP[a * OO * 3 + (o / OD) * OD * 3 + b * OD + o % OD] += (double)(pp * kw[o]) * wk;
-        }
+                for (int o = 0; o < OO; o++)
+                    tLP[a * OO * 3 + (o / OD) * OD * 3 + b * OD + o % OD] += (double)(pp * kw[o]) * wk;
+            }
+    }
 }
 
 // Outer accumulation after outer point k (src/integration.cpp:508-537):
 // termLocal += |aT| psi_a(x) psi_b(x) dx_k innerLocal, termNonloc += |aT| psi_a(x) dx_k innerNonloc.
-template <int OD>
+template <int OD, bool WL = true>
 NLFEM_DEV void outer_point(const double* psx, double ak, const double* inL, const double* inN, double* tL,
                            double* tN) {
 #pragma clang fp contract(fast)
@@ -661,7 +670,7 @@ NLFEM_DEV void outer_point(const double* psx, double ak, const double* inL, cons
     for (int a = 0; a < N; a++)
 #pragma unroll
         for (int b = 0; b < N; b++) {
-            tL[N * a + b] += ak * psx[a / OD] * psx[b / OD] * inL[OD * (a % OD) + (b % OD)];
+            if (WL) tL[N * a + b] += ak * psx[a / OD] * psx[b / OD] * inL[OD * (a % OD) + (b % OD)];
             tN[N * a + b] += ak * psx[a / OD] * inN[(a % OD) * OD + OO * (b / OD) + (b % OD)];
         }
 }
@@ -669,7 +678,7 @@ NLFEM_DEV void outer_point(const double* psx, double ak, const double* inL, cons
 // Interior pair: the retriangulation of B(x_k,delta) n bT is bT itself for every k, so
 // the inner points are bT(Py_i) with basis values psiy_i (src/integration.cpp:383-538 with
 // Rdx = 1 and reTriangle = bT).
-template <int KER, int OD, typename T>
+template <int KER, int OD, typename T, bool WL = true, bool WLP = true>
 NLFEM_DEV void pair_interior(const NumArgs& a, const ElemRec& A, const ElemRec& B, double* tL, PairBlocks<OD>& pb) {
     constexpr int OO = OD * OD;
     for (int i = 0; i < 9 * OO; i++) { pb.tN[i] = 0; pb.tLP[i] = 0; }
@@ -687,10 +696,10 @@ NLFEM_DEV void pair_interior(const NumArgs& a, const ElemRec& A, const ElemRec& 
             T K[OO];
             kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
             T psi[3] = {(T)a.q.psiy[3 * i], (T)a.q.psiy[3 * i + 1], (T)a.q.psiy[3 * i + 2]};
-            inner_point<OD, T>(K, (T)(a.q.dy[i] * absB), psi, wk, inL, inN, pb.tLP);
+            inner_point<OD, T, WL, true, WLP>(K, (T)(a.q.dy[i] * absB), psi, wk, inL, inN, pb.tLP);
         }
         double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
-        outer_point<OD>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
+        outer_point<OD, WL>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
     }
 }
 
@@ -703,6 +712,7 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
     for (int i = 0; i < 9 * OO; i++) { pb.tN[i] = 0; pb.tLP[i] = 0; }
     const double absA = A.absDet;
     const InvMap im = invmap(B.E);
+    const PivotLU lu = pivot_lu(B.E);
     for (int k = 0; k < a.q.nPx; k++) {
         double x[2];
         toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
@@ -719,9 +729,14 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
                 toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
                 T K[OO];
                 kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
-                const double d0 = y[0] - im.e0, d1 = y[1] - im.e1;
-                const double p0 = im.i00 * d0 + im.i01 * d1;
-                const double p1 = im.i10 * d0 + im.i11 * d1;
+                double p0, p1;
+                if constexpr (sizeof(T) == sizeof(double)) {
+                    lu.solve(y[0], y[1], p0, p1);  // the reference's pivoted toRef, bit for bit
+                } else {
+                    const double d0 = y[0] - im.e0, d1 = y[1] - im.e1;
+                    p0 = im.i00 * d0 + im.i01 * d1;
+                    p1 = im.i10 * d0 + im.i11 * d1;
+                }
                 T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
                 inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, wk, inL, inN, pb.tLP);
             }
@@ -908,6 +923,179 @@ __global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
         if (!labA[i]) continue;
         const long long p = a.posTT[9 * aT + 3 * i + j];
         atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
+}
+
+// ======================================================================= interior pairs, owner computes
+// One block per element T.  Every interior pair is integrated twice, once by each of its
+// elements: as outer element T supplies termLocal and termNonloc (rows of T's nodes), as
+// inner element T supplies termLocalPrime and termNonlocPrime (also rows of T's nodes,
+// src/Cassemble.cpp:751-768).  All four land in LDS accumulators keyed by column (one slot
+// per column of the union of T's three rows); the block then writes T's three partial rows
+// with plain coalesced stores into P, and k_merge adds the partial rows of the elements
+// around each node.  No global atomics.
+constexpr unsigned long long ERR_HASH = 128ull;
+
+template <int KER, int OD, typename T>
+__global__ __launch_bounds__(256) void k_interior(NumArgs a) {
+    constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    extern __shared__ double ldsd[];
+    double* acc = ldsd;                                  // [HN][3][OO]
+    int* hk = (int*)(acc + (size_t)a.HN * 3 * OO);       // [HN]
+    __shared__ double red[4][2 * NB];
+    const int T0 = blockIdx.x;
+    const int tid = threadIdx.x;
+    const ElemRec A = a.elem[T0];
+    if (A.label == 0) return;
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, T0, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    if (!(labA[0] || labA[1] || labA[2])) return;
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    for (int i = tid; i < a.HN * 3 * OO; i += 256) acc[i] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        if (!labA[j]) continue;
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256)
+            if (lds_slot(hk, a.HN, a.col[r0 + i]) < 0) atomicOr(&a.counters[CNT_ERROR], ERR_HASH);
+    }
+    __syncthreads();
+    double tL[NB], tLP[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) { tL[i] = 0; tLP[i] = 0; }
+    // T as outer element: front segment, interior class
+    const int nF = a.cntF[T0];
+    for (int p = tid; p < nF; p += 256) {
+        const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
+        if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+        const int U = (int)(e & IDX_MASK);
+        const ElemRec B = a.elem[U];
+        PairBlocks<OD> pb;
+        pair_interior<KER, OD, T, true, false>(a, A, B, tL, pb);
+        for (int j = 0; j < 3; j++) {
+            const int s = lds_find(hk, a.HN, node_of(a.tri, U, j, a.is_DG));
+            for (int i = 0; i < 3; i++) {
+                if (!labA[i]) continue;
+                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], -pb.tN[N * (i * OD + ao) + j * OD + bo]);
+            }
+        }
+    }
+    // T as inner element: pairs (U, T) with T in the interior front segment of U
+    const long long tb = a.tptr[T0], te = a.tptr[T0 + 1];
+    for (long long q = tb + tid; q < te; q += 256) {
+        const uint32_t e = (uint32_t)a.tadj[q];
+        if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+        const int U = (int)(e & IDX_MASK);
+        const ElemRec B = a.elem[U];
+        PairBlocks<OD> pb;
+        pair_interior<KER, OD, T, false, true>(a, B, A, nullptr, pb);
+#pragma unroll
+        for (int t = 0; t < NB; t++) tLP[t] += pb.tLP[t];
+        for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i)
+            const int s = lds_find(hk, a.HN, node_of(a.tri, U, i, a.is_DG));
+            for (int j = 0; j < 3; j++) {
+                if (!labA[j]) continue;
+                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -pb.tN[N * (i * OD + ao) + j * OD + bo]);
+            }
+        }
+    }
+    // block reduction of termLocal and termLocalPrime
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        double v = tL[i], u = tLP[i];
+        for (int m = 32; m > 0; m >>= 1) { v += shfl_xor_d(v, m); u += shfl_xor_d(u, m); }
+        if (lane == 0) { red[w][i] = v; red[w][NB + i] = u; }
+    }
+    __syncthreads();
+    for (int t = tid; t < 2 * NB; t += 256) {
+        const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        const int e = t % NB, ra = e / N, cb = e % N;
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (t < NB) {  // +termLocal into (T_i, T_j)
+            if (!labA[i]) continue;
+            const int s = lds_find(hk, a.HN, rA[j]);
+            if (s >= 0) atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], v);
+        } else {       // +termLocalPrime[N a + b] into (T_b, T_a): row T_j comp bo, col T_i comp ao
+            if (!labA[j]) continue;
+            const int s = lds_find(hk, a.HN, rA[i]);
+            if (s >= 0) atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], v);
+        }
+    }
+    __syncthreads();
+    // partial rows of T's nodes, in column order
+    for (int j = 0; j < 3; j++) {
+        if (!labA[j]) continue;
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        double* dst = a.P + a.poff[3 * T0 + j] * OO;
+        for (int i = tid; i < len; i += 256) {
+            const int s = lds_find(hk, a.HN, a.col[r0 + i]);
+#pragma unroll
+            for (int o = 0; o < OO; o++) dst[(long long)i * OO + o] = (s >= 0) ? acc[(s * 3 + j) * OO + o] : 0.0;
+        }
+    }
+}
+
+// Partial-row offsets: plen[3T+i] = row length of node(T,i) when T takes part in assembly
+// and the node's row is assembled, else 0.
+__global__ void k_plen(int nE, const ElemRec* elem, const int* tri, const int* vlabel, int is_DG,
+                       const long long* rowptr, int* plen) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const bool act = elem[e].label != 0;
+    for (int j = 0; j < 3; j++) {
+        const int r = node_of(tri, e, j, is_DG);
+        plen[3 * e + j] = (act && row_labelled(vlabel, r, is_DG)) ? (int)(rowptr[r + 1] - rowptr[r]) : 0;
+    }
+}
+
+// Node row r = sum of the partial rows of the elements holding node r.
+__global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, const ElemRec* elem, const int* tri,
+                                               const int* vlabel, const long long* vptr, const int* vel,
+                                               const long long* rowptr, const long long* poff, const double* P,
+                                               double* val) {
+    const int r = blockIdx.x;
+    if (!row_labelled(vlabel, r, is_DG)) return;
+    const long long r0 = rowptr[r];
+    const long long len = (rowptr[r + 1] - r0) * oo;
+    __shared__ long long base[MAX_DUAL + 2];
+    __shared__ int nb;
+    if (threadIdx.x == 0) {
+        int n = 0;
+        if (is_DG) {
+            const int T = r / 3;
+            if (elem[T].label != 0) base[n++] = poff[r] * oo;
+        } else {
+            for (long long q = vptr[r]; q < vptr[r + 1] && n < MAX_DUAL + 2; q++) {
+                const int T = vel[q];
+                if (elem[T].label == 0) continue;
+                const int li = tri[3 * T] == r ? 0 : (tri[3 * T + 1] == r ? 1 : 2);
+                base[n++] = poff[3 * T + li] * oo;
+            }
+        }
+        nb = n;
+    }
+    __syncthreads();
+    for (long long p = threadIdx.x; p < len; p += 256) {
+        double v = 0.0;
+        for (int t = 0; t < nb; t++) v += P[base[t] + p];
+        val[r0 * oo + p] = v;
     }
 }
 
@@ -1320,10 +1508,17 @@ void plan_assemble(Plan& P, const Estimates& est) {
     ba.fullCC = c.fullCC;
     ba.cntF = nullptr;
     ba.closeCap = closeCap;
+    // LDS sizing: the visited hash holds ~1.05x the elements of the delta+2h disc (load <= ~0.65
+    // after rounding up to a power of two), the level buffers ~8 sqrt(disc) (a ring of the
+    // disc).  Small tables keep ~13 single-wave blocks per CU resident; overflow of any table is
+    // detected and the BFS is rerun with everything doubled.
+    long long hvis = c.fullCC ? (long long)c.nE : (long long)(1.05 * area_est) + 64;
+    long long fvis = (long long)(8.0 * std::sqrt(area_est)) + 64;
+    if (P.bfs_visit > 0) { visit = P.bfs_visit; hvis = P.bfs_hvis; fvis = P.bfs_fvis; }  // sizes that worked before
     for (int attempt = 0;; attempt++) {
         long long cap = std::max<long long>(visit, 16);
-        long long H = next_pow2(2 * cap);
-        long long F = std::min<long long>(cap, 1 << 16);
+        long long H = next_pow2(std::min<long long>(std::max<long long>(hvis, 64), 2 * cap));
+        long long F = std::min<long long>(next_pow2(std::max<long long>(fvis, 256)), next_pow2(cap));
         size_t ldsb = sizeof(int) * (size_t)(H + 2 * F);
         if (ldsb > 160 * 1024) {
             // too large for LDS: fall back to the largest table that fits and rely on overflow detection
@@ -1344,10 +1539,15 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ev_end(P, PH_BFS);
         read_counters(P, h);
         if (h[CNT_ERROR]) throw std::runtime_error("mesh error (dual graph row longer than 128 elements)");
-        if (!h[CNT_OVERFLOW]) break;
+        if (!h[CNT_OVERFLOW]) {
+            P.bfs_visit = visit; P.bfs_hvis = hvis; P.bfs_fvis = fvis;
+            break;
+        }
         if (attempt >= 3 || (H >= 16384 && ldsb > 64 * 1024 && visit >= c.nE))
             throw std::length_error("BFS capacity overflow: interaction horizon too large for this build");
         visit = std::min<long long>(2 * visit, c.nE);
+        hvis = std::min<long long>(2 * hvis, 2 * (long long)c.nE);
+        fvis *= 2;
     }
     P.stats.pairs = h[CNT_PAIRS];
     P.stats.probes = h[CNT_PROBES];
@@ -1448,6 +1648,60 @@ void plan_assemble(Plan& P, const Estimates& est) {
     na.HN = (HN * 16 <= 64 * 1024) ? (int)HN : 0;
     const size_t nlds = na.HN > 0 ? (size_t)na.HN * 16 : 0;
 
+    // interior pairs: owner computes into partial rows, then merge (plain stores into val)
+    ev_begin(P, PH_INTERIOR);
+    {
+        int* d_plen = nullptr;
+        dalloc(d_plen, 3 * (size_t)c.nE);
+        dalloc(P.d_poff, 3 * (size_t)c.nE + 1);
+        k_plen<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_elem, P.d_tri, P.d_vlabel, c.is_DG, P.d_rowptr, d_plen);
+        scan_exclusive<int>(d_plen, P.d_poff, 3LL * c.nE, scan_scratch(P), st);
+        long long ptotal = 0;
+        HIPCHECK(hipMemcpyAsync(&ptotal, P.d_poff + 3LL * c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        (void)hipFree(d_plen);
+        if (P.ptotal < ptotal * P.oo) {
+            dalloc(P.d_P, (size_t)ptotal * P.oo);
+            P.ptotal = ptotal * P.oo;
+        }
+        na.tptr = P.d_tptr; na.tadj = P.d_tadj; na.poff = P.d_poff; na.P = P.d_P;
+        long long hn = next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+        for (int attempt = 0;; attempt++) {
+            NumArgs ni = na;
+            ni.HN = (int)hn;
+            const size_t lb = (size_t)hn * (24 * P.oo + 4);
+            if (lb > 160 * 1024) throw std::length_error("interior LDS table exceeds 160 KiB");
+#define LAUNCH_INT(KER, OD)                                                                                         \
+    do {                                                                                                            \
+        if (c.precision == 1) {                                                                                     \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_interior<KER, OD, float><<<c.nE, 256, lb, st>>>(ni);                                                  \
+        } else {                                                                                                    \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_interior<KER, OD, double><<<c.nE, 256, lb, st>>>(ni);                                                 \
+        }                                                                                                           \
+    } while (0)
+            switch (c.kernel) {
+                case KER_CONSTANT: LAUNCH_INT(KER_CONSTANT, 1); break;
+                case KER_PARABOLA: LAUNCH_INT(KER_PARABOLA, 1); break;
+                case KER_FRACTIONAL: LAUNCH_INT(KER_FRACTIONAL, 1); break;
+                case KER_MICRO: LAUNCH_INT(KER_MICRO, 1); break;
+                default: LAUNCH_INT(KER_MICRO_FIELD, 2); break;
+            }
+#undef LAUNCH_INT
+            HIPCHECK(hipGetLastError());
+            read_counters(P, h);
+            if (!(h[CNT_ERROR] & ERR_HASH)) break;
+            if (attempt >= 2) throw std::length_error("interior LDS column table overflow");
+            HIPCHECK(hipMemsetAsync(P.d_counters + CNT_ERROR, 0, sizeof(unsigned long long), st));
+            hn *= 2;
+        }
+        k_merge<<<P.nNodes, 256, 0, st>>>(P.nNodes, P.oo, c.is_DG, P.d_elem, P.d_tri, P.d_vlabel, P.d_vptr, P.d_vel,
+                                          P.d_rowptr, P.d_poff, P.d_P, P.d_val);
+        HIPCHECK(hipGetLastError());
+    }
+    ev_end(P, PH_INTERIOR);
+
     if (nclose > 0) {
         ev_begin(P, PH_CLOSE);
         CloseArgs ca;
@@ -1487,10 +1741,6 @@ void plan_assemble(Plan& P, const Estimates& est) {
             default: LAUNCH_PAIRS(KER_MICRO_FIELD, 2, SEG); break;                 \
         }                                                                          \
     } while (0)
-    ev_begin(P, PH_INTERIOR);
-    DISPATCH_PAIRS(0);
-    HIPCHECK(hipGetLastError());
-    ev_end(P, PH_INTERIOR);
     ev_begin(P, PH_CUT);
     DISPATCH_PAIRS(1);
     HIPCHECK(hipGetLastError());
@@ -1537,7 +1787,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
 void plan_free(Plan& P) {
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
-                    P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp};
+                    P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
+                    P.d_poff, P.d_P};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < 2 * PH_COUNT; i++)

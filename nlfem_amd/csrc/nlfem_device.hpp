@@ -99,6 +99,32 @@ NLFEM_DEV InvMap invmap(const double* E) {
     return im;
 }
 
+// toRef of src/mathhelpers.cpp:227-240 split into its per-element part (pivot choice, l, u)
+// and its per-point part, which rounds exactly as solve2x2 (src/mathhelpers.cpp:41-71).
+struct PivotLU {
+    double e0, e1, l, u, a0, a1;  // a0 = A[2 dx0], a1 = A[2 dx0 + 1]
+    int dx0;
+    NLFEM_DEV void solve(double x0, double x1, double& p0, double& p1) const {
+        const double b0 = x0 - e0, b1 = x1 - e1;
+        const double bd0 = dx0 ? b1 : b0, bd1 = dx0 ? b0 : b1;
+        p1 = (bd1 - l * bd0) / u;
+        p0 = (bd0 - a1 * p1) / a0;
+    }
+};
+NLFEM_DEV PivotLU pivot_lu(const double* E) {
+    const double A[4] = {E[2] - E[0], E[4] - E[0], E[3] - E[1], E[5] - E[1]};
+    PivotLU r;
+    r.dx0 = absolute(A[0]) < absolute(A[2]) ? 1 : 0;
+    const int dx1 = 1 - r.dx0;
+    r.e0 = E[0];
+    r.e1 = E[1];
+    r.a0 = A[2 * r.dx0];
+    r.a1 = A[2 * r.dx0 + 1];
+    r.l = A[2 * dx1] / A[2 * r.dx0];
+    r.u = A[2 * dx1 + 1] - r.l * A[2 * r.dx0 + 1];
+    return r;
+}
+
 // ----------------------------------------------------------------------------------------
 // method_retriangulate (src/integration.cpp:1393-1512) with placePointOnCap (:1353-1377),
 // inTriangle (:1283-1298) and rightNormal (src/mathhelpers.cpp:74-78), restated as a

@@ -82,6 +82,9 @@ struct Plan {
     int* d_tpos = nullptr;          // [nnz_nodes] position of the transposed entry
     int* d_posTT = nullptr;         // [nE*9] positions of (node(T,i), node(T,j))
     double* d_val = nullptr;        // [nnz_nodes*oo]
+    long long* d_poff = nullptr;    // [3 nE + 1] offsets of the partial rows of (element, local node)
+    double* d_P = nullptr;          // partial rows of interior pairs (owner computes)
+    long long ptotal = 0;           // allocated doubles in d_P
     long long nnz_nodes = 0;
     int maxRow = 0;
     // Duffy tables [case][traffo][k]: alpha (4), alphaCanceled (4), det (1)
@@ -95,6 +98,8 @@ struct Plan {
     // scratch
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+
+    long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
 
     Stats stats;
     bool assembled = false;
