@@ -233,10 +233,18 @@ __global__ void k_dual(int nE, const int* tri, const long long* vptr, const int*
     if (over) atomicOr(&counters[CNT_ERROR], 2ull);
     if (!FILL) {
         acnt[e] = n;
+        atomicMax(&counters[CNT_MAXT], (unsigned long long)n);
     } else {
         long long o = xadj[e];
         for (int q = 0; q < n; q++) adj[o + q] = buf[q];
     }
+}
+
+__global__ void k_adjpad(int nE, const long long* xadj, const int* adj, int W, int* adjp) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const long long b = xadj[e], d = xadj[e + 1] - b;
+    for (int j = 0; j < W; j++) adjp[(size_t)e * W + j] = j < d ? adj[b + j] : -1;
 }
 
 // ======================================================================= BFS
@@ -245,6 +253,8 @@ struct BfsArgs {
     const ElemRec* elem;
     const long long* xadj;
     const int* adj;
+    const int* adjp;  // dual graph padded to W4*4 entries per row (-1 = none), or null
+    int W4;           // int4 chunks per padded row (0: use the CSR)
     QuadArgs q;
     double sqdelta, sq_in;
     int caps, remote, level0, fullCC;
@@ -259,29 +269,67 @@ struct BfsArgs {
 
 // Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
 // Returns the pair class or -1 when the pair does not interact.
-NLFEM_DEV int classify(const BfsArgs& a, const ElemRec& A, const ElemRec& B, int method) {
+// Outer quadrature points of aT and a bounding circle of them, shared by the wavefront.
+struct OuterPts {
+    double x[2 * MAXQ];
+    double c0, c1;  // centre of the outer points
+    double rin2;    // squared radius around c inside which a vertex is in every ball (with margin), or -1
+    double rout;    // an inner element whose bounding circle stays farther than this from c meets no ball
+};
+
+NLFEM_DEV void outer_points(const BfsArgs& a, const ElemRec& A, OuterPts& op) {
+    double c0 = 0, c1 = 0;
+    for (int k = 0; k < a.q.nPx; k++) {
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], &op.x[2 * k]);
+        c0 += op.x[2 * k];
+        c1 += op.x[2 * k + 1];
+    }
+    c0 /= a.q.nPx;
+    c1 /= a.q.nPx;
+    double r2 = 0;
+    for (int k = 0; k < a.q.nPx; k++) r2 = fmax(r2, sqdist(op.x[2 * k], op.x[2 * k + 1], c0, c1));
+    const double rx = sqrt(r2) * (1 + 1e-12) + 1e-300;
+    const double delta = sqrt(a.sqdelta);
+    const double rin = delta * (1 - 1e-7) - rx;
+    op.c0 = c0;
+    op.c1 = c1;
+    op.rin2 = rin > 0 ? rin * rin : -1.0;
+    op.rout = delta * (1 + 1e-6) + rx;
+}
+
+// Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
+// Returns the pair class or -1 when the pair does not interact.  Two conservative pre-tests
+// settle most candidates without a retriangulation; their margins (1e-7, 1e-6 relative) are
+// far above rounding, so they only ever agree with the exact tests below:
+//  - every vertex of bT within delta(1-1e-7) - r of the outer points' centre: interior;
+//  - the bounding circle of bT farther than delta(1+1e-6) + r: every vertex is outside every
+//    ball and no edge reaches one, so the reference's polygon is empty for every k.
+NLFEM_DEV int classify(const BfsArgs& a, const OuterPts& op, const ElemRec& B, int method) {
     if (method == 2) return (int)CLS_CLOSE;
     if (method == M_BARYCENTER) {
-        for (int k = 0; k < a.q.nPx; k++) {
-            double x[2];
-            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
-            if (bary_interacts(x[0], x[1], B.E, a.sqdelta)) return 3;  // baryCenter pair
-        }
+        for (int k = 0; k < a.q.nPx; k++)
+            if (bary_interacts(op.x[2 * k], op.x[2 * k + 1], B.E, a.sqdelta)) return 3;  // baryCenter pair
         return a.fullCC ? 3 : -1;
     }
-    bool interior = true;
-    for (int k = 0; k < a.q.nPx && interior; k++) {
-        double x[2];
-        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
-        for (int j = 0; j < 3; j++)
-            if (!(sqdist(B.E[2 * j], B.E[2 * j + 1], x[0], x[1]) <= a.sq_in)) interior = false;
+    double dmax = 0;
+    for (int j = 0; j < 3; j++) dmax = fmax(dmax, sqdist(B.E[2 * j], B.E[2 * j + 1], op.c0, op.c1));
+    if (dmax <= op.rin2) return (int)CLS_INTERIOR;
+    {
+        const double b0 = (B.E[0] + B.E[2] + B.E[4]) / 3.0, b1 = (B.E[1] + B.E[3] + B.E[5]) / 3.0;
+        double rb2 = 0;
+        for (int j = 0; j < 3; j++) rb2 = fmax(rb2, sqdist(B.E[2 * j], B.E[2 * j + 1], b0, b1));
+        const double rb = sqrt(rb2) * (1 + 1e-12);
+        if (sqrt(sqdist(b0, b1, op.c0, op.c1)) - rb > op.rout) return a.fullCC ? (int)CLS_CUT : -1;
     }
+    bool interior = true;
+    for (int k = 0; k < a.q.nPx && interior; k++)
+        for (int j = 0; j < 3; j++)
+            if (!(sqdist(B.E[2 * j], B.E[2 * j + 1], op.x[2 * k], op.x[2 * k + 1]) <= a.sq_in)) interior = false;
     if (interior) return (int)CLS_INTERIOR;
     for (int k = 0; k < a.q.nPx; k++) {
-        double x[2];
-        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
         NoEmit ne;
-        if (retriangulate<true>(x[0], x[1], B.E, a.sqdelta, a.caps != 0, ne) >= 3) return (int)CLS_CUT;
+        if (retriangulate<true>(op.x[2 * k], op.x[2 * k + 1], B.E, a.sqdelta, a.caps != 0, ne) >= 3)
+            return (int)CLS_CUT;
     }
     return a.fullCC ? (int)CLS_CUT : -1;
 }
@@ -297,6 +345,7 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
     int* next = test + a.F;
     __shared__ int nTest, nNext, nTot, nF, nB, ovf;
     __shared__ unsigned long long st[6];
+    __shared__ OuterPts op;
     const int lane = threadIdx.x;
     const int aT = blockIdx.x;
     if (aT >= a.nE) return;
@@ -307,6 +356,7 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
     }
     for (int i = lane; i < a.H; i += 64) hash[i] = -1;
     if (lane == 0) {
+        outer_points(a, A, op);
         nTest = 0; nNext = 0; nTot = 0; nF = 0; nB = 0; ovf = 0;
         for (int i = 0; i < 6; i++) st[i] = 0;
     }
@@ -338,7 +388,7 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
             if (B.label == 0) {
                 push = true;  // label-0 elements are traversed only (src/Cassemble.cpp:773-775)
             } else {
-                int cls = classify(a, A, B, method);
+                int cls = classify(a, op, B, method);
                 if (cls >= 0) {
                     push = true;
                     int t = atomicAdd(&nTot, 1);
@@ -372,17 +422,34 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
         __syncthreads();
         // discover the next level: unvisited neighbours of the expanded elements
         const int nn = nNext;
+        auto visit = [&](int e) {
+            int r = lds_insert(hash, a.H, e);
+            if (r < 0) ovf = 1;
+            else if (r == 1) {
+                int q = atomicAdd(&nTest, 1);
+                if (q < a.F) test[q] = e;
+                else ovf = 1;
+            }
+        };
         for (int j = lane; j < nn; j += 64) {
             const int s = next[j];
-            for (long long p = a.xadj[s]; p < a.xadj[s + 1]; p++) {
-                int e = a.adj[p];
-                int r = lds_insert(hash, a.H, e);
-                if (r < 0) { ovf = 1; break; }
-                if (r == 1) {
-                    int q = atomicAdd(&nTest, 1);
-                    if (q < a.F) test[q] = e;
-                    else ovf = 1;
+            if (a.W4 > 0) {
+                // padded row: all loads issued before the first insert
+                const int4* row = reinterpret_cast<const int4*>(a.adjp) + (size_t)s * a.W4;
+                int4 v[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    if (c < a.W4) v[c] = row[c];
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    if (c >= a.W4) break;
+                    if (v[c].x >= 0) visit(v[c].x);
+                    if (v[c].y >= 0) visit(v[c].y);
+                    if (v[c].z >= 0) visit(v[c].z);
+                    if (v[c].w >= 0) visit(v[c].w);
                 }
+            } else {
+                for (long long p = a.xadj[s]; p < a.xadj[s + 1]; p++) visit(a.adj[p]);
             }
         }
         __syncthreads();
@@ -630,8 +697,10 @@ struct PairBlocks {
 // Contribution of one inner quadrature point y (with weight w = dy_i * |det rT|, basis values
 // psi at y, kernel K = k(x_k, y)) to the inner sums of outer point k, and to termLocalPrime.
 // Follows src/integration.cpp:424-476.
+// sLP holds the six symmetric products psi_a psi_b (a <= b) times k w, summed over the inner
+// points of one outer point; add_lp scales them by dx_k |aT| into termLocalPrime.
 template <int OD, typename T, bool WL = true, bool WN = true, bool WLP = true>
-NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double wk, double* inL, double* inN, double* tLP) {
+NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double* inL, double* inN, double* sLP) {
 #pragma clang fp contract(fast)
     constexpr int OO = OD * OD;
     T kw[OO];
@@ -647,16 +716,30 @@ NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double wk, double* inL
             for (int o = 0; o < OO; o++) inN[b * OO + o] += (double)(psi[b] * kw[o]);
     }
     if (WLP) {
+        const T pp[6] = {psi[0] * psi[0], psi[0] * psi[1], psi[0] * psi[2], psi[1] * psi[1], psi[1] * psi[2],
+                         psi[2] * psi[2]};
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+        for (int q = 0; q < 6; q++)
 #pragma unroll
-            for (int b = 0; b < 3; b++) {
-                T pp = psi[a] * psi[b];
-#pragma unroll
-                for (int o = 0; o < OO; o++)
-                    tLP[a * OO * 3 + (o / OD) * OD * 3 + b * OD + o % OD] += (double)(pp * kw[o]) * wk;
-            }
+            for (int o = 0; o < OO; o++) sLP[q * OO + o] += (double)(pp[q] * kw[o]);
     }
+}
+
+NLFEM_DEV constexpr int sym6(int a, int b) { return a <= b ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : sym6(b, a); }
+
+// termLocalPrime[a*OO*3 + (o/OD)*OD*3 + b*OD + o%OD] += psi_a psi_b k'[o] dy rTdet dx_k |aT|
+// (src/integration.cpp:462-476), with the inner sum taken first.
+template <int OD>
+NLFEM_DEV void add_lp(const double* sLP, double wk, double* tLP) {
+#pragma clang fp contract(fast)
+    constexpr int OO = OD * OD;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int o = 0; o < OO; o++)
+                tLP[a * OO * 3 + (o / OD) * OD * 3 + b * OD + o % OD] += wk * sLP[sym6(a, b) * OO + o];
 }
 
 // Outer accumulation after outer point k (src/integration.cpp:508-537):
@@ -690,14 +773,17 @@ NLFEM_DEV void pair_interior(const NumArgs& a, const ElemRec& A, const ElemRec& 
         for (int o = 0; o < OO; o++) inL[o] = 0;
         for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
         const double wk = a.q.dx[k] * absA;
+        double sLP[6 * OO];
+        for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
         for (int i = 0; i < a.q.nPy; i++) {
             double y[2];
             toPhys(B.E, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
             T K[OO];
             kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
             T psi[3] = {(T)a.q.psiy[3 * i], (T)a.q.psiy[3 * i + 1], (T)a.q.psiy[3 * i + 2]};
-            inner_point<OD, T, WL, true, WLP>(K, (T)(a.q.dy[i] * absB), psi, wk, inL, inN, pb.tLP);
+            inner_point<OD, T, WL, true, WLP>(K, (T)(a.q.dy[i] * absB), psi, inL, inN, sLP);
         }
+        if (WLP) add_lp<OD>(sLP, wk, pb.tLP);
         double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
         outer_point<OD, WL>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
     }
@@ -720,6 +806,8 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
         for (int o = 0; o < OO; o++) inL[o] = 0;
         for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
         const double wk = a.q.dx[k] * absA;
+        double sLP[6 * OO];
+        for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
         auto emit = [&](double r0, double r1, double s0, double s1, double t0, double t1) {
             const double rT[6] = {r0, r1, s0, s1, t0, t1};
             const double rTdet = absDet(rT);
@@ -738,10 +826,11 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
                     p1 = im.i10 * d0 + im.i11 * d1;
                 }
                 T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
-                inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, wk, inL, inN, pb.tLP);
+                inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, inL, inN, sLP);
             }
         };
         retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, emit);
+        add_lp<OD>(sLP, wk, pb.tLP);
         double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
         outer_point<OD>(psx, absA * a.q.dx[k], inL, inN, tL, pb.tN);
     }
@@ -770,7 +859,7 @@ NLFEM_DEV void pair_bary(const NumArgs& a, const ElemRec& A, const ElemRec& B, d
                 T K[OO];
                 kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
                 T psi[3] = {(T)a.q.psiy[3 * i], (T)a.q.psiy[3 * i + 1], (T)a.q.psiy[3 * i + 2]};
-                inner_point<OD, T>(K, (T)(a.q.dy[i] * absB), psi, 0.0, inL, inN, dummy);
+                inner_point<OD, T, true, true, false>(K, (T)(a.q.dy[i] * absB), psi, inL, inN, dummy);
             }
         }
         double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
@@ -1096,6 +1185,233 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
         double v = 0.0;
         for (int t = 0; t < nb; t++) v += P[base[t] + p];
         val[r0 * oo + p] = v;
+    }
+}
+
+// ======================================================================= cut pairs, two phases
+// One block of 256 threads per outer element aT, cut pairs in chunks of CUT_ITEMS / nPx pairs.
+//   A  one thread per (pair, outer point k): streamed retriangulation; the polygon points go
+//      to an LDS list (at most 9 points, as in the reference's R[18]).
+//   B  one thread per fan triangle (R0, R_j, R_j+1) of any item of the chunk: the inner
+//      rule on the triangle; innerLocal / innerNonloc of the item and termLocalPrime of the
+//      pair accumulate in LDS (FP64 atomics).  Every thread has the same work (nPy points),
+//      whatever the polygon sizes.
+//   C  one thread per item: outer accumulation (src/integration.cpp:508-537) into termLocal
+//      (registers) and termNonloc of the pair (LDS).
+//   D  scatter of the chunk's pairs, one thread per (pair, i, j).
+constexpr int CUT_ITEMS = 128;
+constexpr int CUT_MAXP = 9;
+constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= CUT_ITEMS / CUT_MAXPAIRS = 4
+
+struct PointCollector {
+    double2* pts;
+    int nt;
+    NLFEM_DEV void operator()(double r0, double r1, double s0, double s1, double t0, double t1) {
+        if (nt == 0) {
+            pts[0] = make_double2(r0, r1);
+            pts[1] = make_double2(s0, s1);
+            pts[2] = make_double2(t0, t1);
+        } else if (nt + 2 < CUT_MAXP) {
+            pts[nt + 2] = make_double2(t0, t1);
+        }
+        nt++;
+    }
+};
+
+template <int KER, int OD, typename T>
+__global__ __launch_bounds__(256) void k_cut(NumArgs a) {
+    constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    extern __shared__ double ldsc[];
+    double2* pts = (double2*)ldsc;                                  // [CUT_ITEMS][CUT_MAXP]
+    double* iacc = (double*)(pts + CUT_ITEMS * CUT_MAXP);          // [CUT_ITEMS][10*OO]: innerLocal, innerNonloc, sLP
+    double* pacc = iacc + CUT_ITEMS * 10 * OO;                      // [CP][2*NB]: termNonloc, termLocalPrime
+    int* hk = (int*)(pacc + CUT_MAXPAIRS * 2 * NB);                 // [HN]
+    int* hv = hk + a.HN;                                            // [HN][3]
+    __shared__ int ntri[CUT_ITEMS];
+    __shared__ int tpre[CUT_ITEMS + 1];
+    __shared__ int pb_[CUT_MAXPAIRS];                               // inner element of each chunk pair
+    __shared__ PivotLU lus[CUT_MAXPAIRS];
+    __shared__ double red[4][NB];
+    __shared__ unsigned subtri;
+    __shared__ int wsum[2];
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nB = a.cntB[aT];
+    if (nB == 0) return;
+    const int nPx = a.q.nPx;
+    const int CP = min(CUT_ITEMS / nPx, CUT_MAXPAIRS);
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    if (tid == 0) subtri = 0;
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256) {
+            int s = lds_slot(hk, a.HN, a.col[r0 + i]);
+            if (s >= 0) hv[3 * s + j] = i;
+            else atomicOr(&a.counters[CNT_ERROR], 16ull);
+        }
+    }
+    double tL[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) tL[i] = 0;
+    const double absA = A.absDet;
+    unsigned mytri = 0;
+    for (int c0 = 0; c0 < nB; c0 += CP) {
+        const int np = min(CP, nB - c0);
+        const int nit = np * nPx;
+        __syncthreads();
+        for (int i = tid; i < CUT_ITEMS * 10 * OO; i += 256) iacc[i] = 0.0;
+        if (tid < np) {
+            const uint32_t e = a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)];
+            const int b = (int)(e & IDX_MASK);
+            pb_[tid] = b;
+            lus[tid] = pivot_lu(a.elem[b].E);
+        }
+        __syncthreads();
+        // A: polygons
+        if (tid < nit) {
+            const int p = tid / nPx, k = tid % nPx;
+            const ElemRec B = a.elem[pb_[p]];
+            double x[2];
+            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+            PointCollector pc{pts + tid * CUT_MAXP, 0};
+            retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
+            if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
+            ntri[tid] = min(pc.nt, CUT_MAXP - 2);
+            mytri += ntri[tid];
+        } else if (tid < CUT_ITEMS) {
+            ntri[tid] = 0;
+        }
+        __syncthreads();
+        {  // inclusive scan of ntri over the first two waves (items < CUT_ITEMS = 128)
+            int v = (tid < CUT_ITEMS) ? ntri[tid] : 0;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(v, off, 64);
+                if (lane >= off) v += y;
+            }
+            if (lane == 63 && w < 2) wsum[w] = v;
+            __syncthreads();
+            if (w == 1) v += wsum[0];
+            if (tid < CUT_ITEMS) tpre[tid + 1] = v;
+            if (tid == 0) tpre[0] = 0;
+        }
+        __syncthreads();
+        // B: fan triangles
+        const int ntot = tpre[nit];
+        for (int t = tid; t < ntot; t += 256) {
+            int lo = 0, hi = nit - 1;  // item with tpre[it] <= t < tpre[it+1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (tpre[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int it = lo, j = t - tpre[it];
+            const int p = it / nPx, k = it % nPx;
+            const double2 R0 = pts[it * CUT_MAXP], R1 = pts[it * CUT_MAXP + j + 1], R2 = pts[it * CUT_MAXP + j + 2];
+            const double rT[6] = {R0.x, R0.y, R1.x, R1.y, R2.x, R2.y};
+            const double rTdet = absDet(rT);
+            double x[2];
+            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+            const double wk = a.q.dx[k] * absA;
+            const PivotLU lu = lus[p];
+            double inL[OO], inN[3 * OO], sLP[6 * OO];
+#pragma unroll
+            for (int o = 0; o < OO; o++) inL[o] = 0;
+#pragma unroll
+            for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+#pragma unroll
+            for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
+            for (int i = 0; i < a.q.nPy; i++) {
+                double y[2];
+                toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                T K[OO];
+                kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+                double p0, p1;
+                lu.solve(y[0], y[1], p0, p1);
+                T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
+                inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, inL, inN, sLP);
+            }
+            double* ia = iacc + it * 10 * OO;  // the ~2.5 triangles of an item share these slots
+#pragma unroll
+            for (int o = 0; o < OO; o++) atomicAdd(&ia[o], inL[o]);
+#pragma unroll
+            for (int o = 0; o < 3 * OO; o++) atomicAdd(&ia[OO + o], inN[o]);
+#pragma unroll
+            for (int o = 0; o < 6 * OO; o++) atomicAdd(&ia[4 * OO + o], sLP[o]);
+        }
+        __syncthreads();
+        // C: outer accumulation (src/integration.cpp:508-537), one thread per pair
+        if (tid < np) {
+            double tN[NB], tLP[NB];
+#pragma unroll
+            for (int o = 0; o < NB; o++) { tN[o] = 0; tLP[o] = 0; }
+            for (int k = 0; k < nPx; k++) {
+                const double* ia = iacc + (tid * nPx + k) * 10 * OO;
+                double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+                outer_point<OD>(psx, absA * a.q.dx[k], ia, ia + OO, tL, tN);
+                add_lp<OD>(ia + 4 * OO, a.q.dx[k] * absA, tLP);
+            }
+            double* pa = pacc + tid * 2 * NB;
+#pragma unroll
+            for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
+        }
+        __syncthreads();
+        // D: scatter (src/Cassemble.cpp:721-770): thread per (pair, i, j)
+        for (int t = tid; t < np * 9; t += 256) {
+            const int p = t / 9, i = (t % 9) / 3, j = t % 3;
+            const int b = pb_[p];
+            const double* tN = pacc + p * 2 * NB;
+            const double* tLP = tN + NB;
+            const int rBj = node_of(a.tri, b, j, a.is_DG);
+            const bool labBj = row_labelled(a.vlabel, rBj, a.is_DG);
+            const int s = lds_find(hk, a.HN, rBj);
+            if (s < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
+            const long long pos = a.rowptr[rA[i]] + hv[3 * s + i];  // entry (aAdx[i], bAdx[j])
+#pragma unroll
+            for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                for (int bo = 0; bo < OD; bo++) {
+                    const double v = tN[N * (i * OD + ao) + j * OD + bo];
+                    if (labA[i]) atomicAdd(&a.val[pos * OO + ao * OD + bo], -v);
+                    if (labBj) atomicAdd(&a.val[(long long)a.tpos[pos] * OO + bo * OD + ao], -v);
+                }
+            if (labBj) {  // +termLocalPrime[N (i OD+ao) + j OD+bo] into (bAdx[j], bAdx[i])
+                const long long q = a.posTT[9 * b + 3 * j + i];
+#pragma unroll
+                for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                    for (int bo = 0; bo < OD; bo++)
+                        atomicAdd(&a.val[q * OO + bo * OD + ao], tLP[N * (i * OD + ao) + j * OD + bo]);
+            }
+        }
+    }
+    // termLocal: block reduction, then into (aAdx[i], aAdx[j])
+    atomicAdd(&subtri, mytri);
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        double v = tL[i];
+        for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+        if (lane == 0) red[w][i] = v;
+    }
+    __syncthreads();
+    if (tid == 0) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)subtri);
+    for (int t = tid; t < NB; t += 256) {
+        const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        const int ra = t / N, cb = t % N;
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (!labA[i]) continue;
+        const long long p = a.posTT[9 * aT + 3 * i + j];
+        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
     }
 }
 
@@ -1487,6 +1803,15 @@ void plan_assemble(Plan& P, const Estimates& est) {
     k_dual<true><<<nblk(c.nE, 128), 128, 0, st>>>(c.nE, P.d_tri, P.d_vptr, P.d_vel, nullptr, P.d_xadj, P.d_adj,
                                                   P.d_counters);
     HIPCHECK(hipGetLastError());
+    read_counters(P, h);
+    const int maxdeg = (int)h[CNT_MAXT];
+    int W4 = (maxdeg + 3) / 4;
+    if (W4 > 8) W4 = 0;  // very irregular meshes: BFS walks the CSR rows
+    if (W4 > 0) {
+        dalloc(P.d_adjp, (size_t)c.nE * 4 * W4);
+        k_adjpad<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_xadj, P.d_adj, 4 * W4, P.d_adjp);
+        HIPCHECK(hipGetLastError());
+    }
     ev_end(P, PH_DUAL);
 
     // ---------------------------------------------------------------- BFS
@@ -1500,6 +1825,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     dalloc(P.d_closeList, (size_t)closeCap);
     BfsArgs ba;
     ba.nE = c.nE; ba.elem = P.d_elem; ba.xadj = P.d_xadj; ba.adj = P.d_adj; ba.q = c.q;
+    ba.adjp = P.d_adjp; ba.W4 = W4;
     ba.sqdelta = c.sqdelta;
     ba.sq_in = c.sqdelta * (1.0 - 1e-9);
     ba.caps = c.caps;
@@ -1742,7 +2068,33 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                          \
     } while (0)
     ev_begin(P, PH_CUT);
-    DISPATCH_PAIRS(1);
+    if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
+        DISPATCH_PAIRS(1);
+    } else {
+        NumArgs nc = na;
+        nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+        const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP +
+                          sizeof(double) * (CUT_ITEMS * 10 * P.oo + CUT_MAXPAIRS * 18 * P.oo) + (size_t)nc.HN * 16;
+        if (lb > 160 * 1024) throw std::length_error("cut-pair LDS exceeds 160 KiB");
+#define LAUNCH_CUT(KER, OD)                                                                                          \
+    do {                                                                                                             \
+        if (c.precision == 1) {                                                                                      \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER, OD, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_cut<KER, OD, float><<<c.nE, 256, lb, st>>>(nc);                                                        \
+        } else {                                                                                                     \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER, OD, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_cut<KER, OD, double><<<c.nE, 256, lb, st>>>(nc);                                                       \
+        }                                                                                                            \
+    } while (0)
+        switch (c.kernel) {
+            case KER_CONSTANT: LAUNCH_CUT(KER_CONSTANT, 1); break;
+            case KER_PARABOLA: LAUNCH_CUT(KER_PARABOLA, 1); break;
+            case KER_FRACTIONAL: LAUNCH_CUT(KER_FRACTIONAL, 1); break;
+            case KER_MICRO: LAUNCH_CUT(KER_MICRO, 1); break;
+            default: LAUNCH_CUT(KER_MICRO_FIELD, 2); break;
+        }
+#undef LAUNCH_CUT
+    }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_CUT);
 #undef DISPATCH_PAIRS
@@ -1788,7 +2140,7 @@ void plan_free(Plan& P) {
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
-                    P.d_poff, P.d_P};
+                    P.d_poff, P.d_P, P.d_adjp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < 2 * PH_COUNT; i++)

@@ -99,16 +99,30 @@ NLFEM_DEV InvMap invmap(const double* E) {
     return im;
 }
 
-// toRef of src/mathhelpers.cpp:227-240 split into its per-element part (pivot choice, l, u)
-// and its per-point part, which rounds exactly as solve2x2 (src/mathhelpers.cpp:41-71).
+// Quotient n/d from the correctly rounded reciprocal r = 1/d and one exact residual step:
+// q = n r, e = fma(-q, d, n) (exact), result = fma(e, r, q).  With r = RN(1/d) this is the
+// correctly rounded quotient (Markstein's correction), i.e. it equals IEEE n/d; it replaces a
+// full FP64 division (about a dozen instructions) by three.
+NLFEM_DEV double div_by_recip(double n, double d, double r) {
+    const double q = n * r;
+    const double e = __builtin_fma(-q, d, n);
+    return __builtin_fma(e, r, q);
+}
+
+// toRef of src/mathhelpers.cpp:227-240 split into its per-element part (pivot choice, l, u,
+// reciprocals of the pivots) and its per-point part, which rounds exactly as solve2x2
+// (src/mathhelpers.cpp:41-71): the numerators are formed without fused operations and the
+// two divisions are correctly rounded.
 struct PivotLU {
-    double e0, e1, l, u, a0, a1;  // a0 = A[2 dx0], a1 = A[2 dx0 + 1]
+    double e0, e1, l, u, a0, a1, ru, ra0;  // a0 = A[2 dx0], a1 = A[2 dx0 + 1], ru = 1/u, ra0 = 1/a0
     int dx0;
     NLFEM_DEV void solve(double x0, double x1, double& p0, double& p1) const {
         const double b0 = x0 - e0, b1 = x1 - e1;
         const double bd0 = dx0 ? b1 : b0, bd1 = dx0 ? b0 : b1;
-        p1 = (bd1 - l * bd0) / u;
-        p0 = (bd0 - a1 * p1) / a0;
+        const double n1 = bd1 - l * bd0;
+        p1 = div_by_recip(n1, u, ru);
+        const double n0 = bd0 - a1 * p1;
+        p0 = div_by_recip(n0, a0, ra0);
     }
 };
 NLFEM_DEV PivotLU pivot_lu(const double* E) {
@@ -122,6 +136,8 @@ NLFEM_DEV PivotLU pivot_lu(const double* E) {
     r.a1 = A[2 * r.dx0 + 1];
     r.l = A[2 * dx1] / A[2 * r.dx0];
     r.u = A[2 * dx1 + 1] - r.l * A[2 * r.dx0 + 1];
+    r.ru = 1.0 / r.u;
+    r.ra0 = 1.0 / r.a0;
     return r;
 }
 

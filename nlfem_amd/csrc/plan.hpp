@@ -66,6 +66,7 @@ struct Plan {
     // dual graph
     long long* d_xadj = nullptr; // [nE+1]
     int* d_adj = nullptr;
+    int* d_adjp = nullptr;       // dual graph rows padded to a multiple of 4 (BFS)
     // neighbour lists (padded: cap slots per outer element; front = interior/close, back = cut/bary)
     int cap = 0;
     uint32_t* d_nbr = nullptr;
