@@ -101,6 +101,23 @@ int nlfem_load_vector(int K, const long* ptrTriangles, const long* ptrLabelTrian
                       const long* ptrLabelVerts, int nE, int nV, const double* Px, int nPx, const double* dx,
                       int is_DiscontinuousGalerkin, const char* str_model_f, int outdim, double* fd);
 
+/* ------------------------------------------------------------------------------------
+ * Cut-pair surrogate network (no counterpart in the reference): a 12 -> 64 -> 64 -> 64 -> 9
+ * perceptron with SiLU activations evaluated with BF16 MFMA and FP32 accumulation.
+ * params[9737] = W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[9][64] b4[9];
+ * norm[42] = input mean[12], input 1/std[12], output mean[9], output std[9].
+ * X[n][12] inputs, T[n][9] targets, Y[n][9] outputs, all host memory.
+ * ------------------------------------------------------------------------------------ */
+int nlfem_mlp_info(int* nin, int* nhidden, int* nout, int* nparam, int* nnorm);
+int nlfem_mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y);
+/* Gradient of the mean squared error in normalised output units over all n samples. */
+int nlfem_mlp_grad(const float* params, const float* norm, const float* X, const float* T, long long n,
+                   float* grad, float* loss);
+/* Adam (warm-up + cosine decay), minibatches of `batch` (multiple of 64) per epoch;
+ * params is updated in place; loss_hist[epochs] (may be NULL) gets the mean minibatch loss. */
+int nlfem_mlp_train(float* params, const float* norm, const float* X, const float* T, long long n, int epochs,
+                    int batch, float lr_max, unsigned seed, float* loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@ _lp = C.POINTER(C.c_long)
 _llp = C.POINTER(C.c_longlong)
 _ip = C.POINTER(C.c_int)
 _cp = C.c_char_p
+_fp = C.POINTER(C.c_float)
 
 # exported symbols and their signatures: (restype, argtypes)
 SIGNATURES = {
@@ -43,6 +44,10 @@ SIGNATURES = {
     "nlfem_plan_destroy": (None, [C.c_void_p]),
     "nlfem_load_vector": (C.c_int, [C.c_int, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _cp,
                                     C.c_int, _dp]),
+    "nlfem_mlp_info": (C.c_int, [_ip, _ip, _ip, _ip, _ip]),
+    "nlfem_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_longlong, _fp]),
+    "nlfem_mlp_grad": (C.c_int, [_fp, _fp, _fp, _fp, C.c_longlong, _fp, _fp]),
+    "nlfem_mlp_train": (C.c_int, [_fp, _fp, _fp, _fp, C.c_longlong, C.c_int, C.c_int, C.c_float, C.c_uint, _fp]),
 }
 
 

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libnlfem_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NLFEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["assembly.hip", "aux_kernels.hip", "capi.cpp"]
+SOURCES = ["assembly.hip", "aux_kernels.hip", "mlp_train.hip", "capi.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

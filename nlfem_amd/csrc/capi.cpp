@@ -22,6 +22,13 @@ void run_mass(int K, const long* tri, const long* lab, const double* verts, cons
               const double* P, int nP, const double* dx, int is_DG, int od, const double* ud, double* vd_host);
 void run_retriangulate(int n, const double* x, const double* TE, double sqdelta, int caps, double* lists, int* ntri);
 void run_geom(int n, int op, const double* E, const double* p, double* out, int* ok);
+namespace mlp {
+void mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y);
+void mlp_grad(const float* params, const float* norm, const float* X, const float* T, long long n, float* grad,
+              float* loss);
+void mlp_train(float* params, const float* norm, const float* X, const float* T, long long n, int epochs, int batch,
+               float lr_max, unsigned seed, float* loss_hist);
+}  // namespace mlp
 }  // namespace nlfem
 
 using namespace nlfem;
@@ -490,3 +497,36 @@ void toRef(const double* E, const double* phys_x, double* ref_p) { die_if(nlfem_
 void toPhys(const double* E, const double* p, double* out_x) { die_if(nlfem_toPhys(E, p, 2, out_x)); }
 void toPhys(const double* E, const double* p, int dim, double* out_x) { die_if(nlfem_toPhys(E, p, dim, out_x)); }
 void solve2x2(const double* A, const double* b, double* x) { die_if(nlfem_solve2x2(A, b, x)); }
+
+// ------------------------------------------------------------------ surrogate network
+extern "C" int nlfem_mlp_info(int* nin, int* nhidden, int* nout, int* nparam, int* nnorm) {
+    if (!nin || !nhidden || !nout || !nparam || !nnorm) return fail(NLFEM_EINVAL, "null pointer");
+    *nin = 12; *nhidden = 64; *nout = 9; *nparam = 9737; *nnorm = 42;
+    return NLFEM_OK;
+}
+
+extern "C" int nlfem_mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y) {
+    if (!params || !norm || (n > 0 && (!X || !Y)) || n < 0) return fail(NLFEM_EINVAL, "invalid arguments");
+    GUARD_BEGIN
+    mlp::mlp_forward(params, norm, X, n, Y);
+    return NLFEM_OK;
+    GUARD_END
+}
+
+extern "C" int nlfem_mlp_grad(const float* params, const float* norm, const float* X, const float* T, long long n,
+                              float* grad, float* loss) {
+    if (!params || !norm || !X || !T || !grad || !loss || n <= 0) return fail(NLFEM_EINVAL, "invalid arguments");
+    GUARD_BEGIN
+    mlp::mlp_grad(params, norm, X, T, n, grad, loss);
+    return NLFEM_OK;
+    GUARD_END
+}
+
+extern "C" int nlfem_mlp_train(float* params, const float* norm, const float* X, const float* T, long long n,
+                               int epochs, int batch, float lr_max, unsigned seed, float* loss_hist) {
+    if (!params || !norm || !X || !T || n <= 0 || !(lr_max > 0)) return fail(NLFEM_EINVAL, "invalid arguments");
+    GUARD_BEGIN
+    mlp::mlp_train(params, norm, X, T, n, epochs, batch, lr_max, seed, loss_hist);
+    return NLFEM_OK;
+    GUARD_END
+}
