@@ -94,6 +94,19 @@ int nlfem_plan_neighbours(const nlfem_plan* plan, long long* ptr, int* nbr, unsi
  * max_list, subtri (fan triangles integrated by the cut kernel), then per-phase milliseconds (dual, bfs, transpose, pattern, positions, close,
  * interior, cut, compact).  n = number of doubles available in out; returns NLFEM_OK. */
 int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
+/* Cut pairs through the surrogate network (no counterpart in the reference; constant kernel,
+ * retriangulate).  nnet = 1 or 2 networks (params[nnet][9737], norm[nnet][42], see the
+ * surrogate section below); nnet = 0 switches it off.  A pair goes to the exact method when
+ * one of its items lies outside [smin, smax] (sqrt|det bT| / delta) or the two networks
+ * disagree by more than tol on an output ratio.  Stats gain (after the phase times) the
+ * pairs taken by the surrogate, the pairs handed back, and the network items evaluated. */
+int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm, float tol,
+                             double smin, double smax);
+/* Training samples from the exact FP64 cut-pair sums of an assembled plan: feat[maxn][12],
+ * targ[maxn][9] (ratios to the full-triangle sums); items kept when a hash of
+ * (aT, bT, k, salt) is a multiple of stride.  *count = qualifying items (may exceed maxn). */
+int nlfem_plan_surrogate_samples(nlfem_plan* plan, long long maxn, unsigned stride, unsigned salt, double smin,
+                                 double smax, float* feat, float* targ, long long* count);
 void nlfem_plan_destroy(nlfem_plan* plan);
 
 /* Load vector on the device (par_forcing, src/Cassemble.cpp:857-928): fd[K]. */

@@ -30,6 +30,7 @@
 
 #include "nlfem_device.hpp"
 #include "plan.hpp"
+#include "mlp.hpp"
 
 namespace nlfem {
 
@@ -42,7 +43,8 @@ namespace nlfem {
     } while (0)
 
 constexpr int CNT_PAIRS = 0, CNT_PROBES = 1, CNT_CLOSE = 2, CNT_INTERIOR = 3, CNT_CUT = 4, CNT_BARY = 5,
-              CNT_OVERFLOW = 6, CNT_ERROR = 7, CNT_MAXLIST = 8, CNT_MAXROW = 9, CNT_MAXT = 10, CNT_SUBTRI = 11;
+              CNT_OVERFLOW = 6, CNT_ERROR = 7, CNT_MAXLIST = 8, CNT_MAXROW = 9, CNT_MAXT = 10, CNT_SUBTRI = 11,
+              CNT_SUR_ACC = 12, CNT_SUR_REJ = 13, CNT_SUR_ITEMS = 14, CNT_SAMPLES = 15;
 
 // ======================================================================= small utilities
 __global__ void k_fill_int(int* p, int v, long long n) {
@@ -643,6 +645,27 @@ __global__ void k_posTT(int nE, const int* tri, int is_DG, const long long* rowp
 }
 
 // ======================================================================= pair quadrature
+// Cut-pair surrogate (mlp.hpp): networks, acceptance rule, exact full-triangle moments of
+// the inner rule, the per-element lists of pairs handed back to the exact method, and the
+// sample buffers of the training-data mode.
+struct SurArgs {
+    const mlp::bf16x4* wf;  // [nnet][NFRAG_FWD * 64]
+    const float* bias;      // [nnet][BIAS_PAD]
+    const float* norm;      // [nnet][NNORM]
+    int nnet;               // networks in the ensemble (1 or 2)
+    float tol;              // largest accepted spread of the ensemble (ratio units)
+    double smin, smax;      // domain: sqrt|det bT| / delta
+    double q0, q1[3], q2[6];  // sum_i dy_i, sum_i dy_i psi_b(P_i), sum_i dy_i psi_a psi_b(P_i) (a <= b)
+    int* rlist;             // rejected pairs: rlist[roff[aT] + r], r < rcnt[aT]
+    const long long* roff;
+    int* rcnt;
+    int list_in;            // the exact cut kernel reads its pairs from rlist / rcnt
+    float* sfeat;           // training samples [smaxn][12] and targets [smaxn][9]
+    float* starg;
+    long long smaxn;
+    unsigned stride, salt;  // keep an item when hash(aT, bT, k, salt) % stride == 0
+};
+
 struct NumArgs {
     int nE, cap, is_DG, caps, HN, od;
     const ElemRec* elem;
@@ -664,6 +687,7 @@ struct NumArgs {
     const int* tadj;
     const long long* poff;
     double* P;
+    SurArgs sur;
 };
 
 // Kernel value(s) at z = x - y: OO entries (row-major od x od).
@@ -926,6 +950,69 @@ NLFEM_DEV void scatter_pair(const NumArgs& a, const int* rA, const bool* labA, i
                         atomicAdd(&a.val[p * OO + bo * OD + ao], pb.tLP[N * (i * OD + ao) + j * OD + bo]);
             }
         }
+    }
+}
+
+
+// Scatter of the blocks of a chunk of cut pairs (src/Cassemble.cpp:721-770), one thread per
+// (pair, i, j) of a 256-thread block: -termNonloc into (aAdx[i], bAdx[j]) and, transposed,
+// into (bAdx[j], aAdx[i]); +termLocalPrime into (bAdx[j], bAdx[i]).  Positions of row aAdx[i]
+// come from the block's LDS column map (hk, hv), transposed entries through tpos, inner
+// element entries through posTT.  Pairs with pbs[p] < 0 are skipped.
+template <int OD>
+NLFEM_DEV void cut_scatter(const NumArgs& a, int np, const int* pbs, const double* pacc, const int* rA,
+                           const bool* labA, const int* hk, const int* hv) {
+    constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    for (int t = threadIdx.x; t < np * 9; t += 256) {
+        const int p = t / 9, i = (t % 9) / 3, j = t % 3;
+        const int b = pbs[p];
+        if (b < 0) continue;
+        const double* tN = pacc + p * 2 * NB;
+        const double* tLP = tN + NB;
+        const int rBj = node_of(a.tri, b, j, a.is_DG);
+        const bool labBj = row_labelled(a.vlabel, rBj, a.is_DG);
+        const int s = lds_find(hk, a.HN, rBj);
+        if (s < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
+        const long long pos = a.rowptr[rA[i]] + hv[3 * s + i];  // entry (aAdx[i], bAdx[j])
+#pragma unroll
+        for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+            for (int bo = 0; bo < OD; bo++) {
+                const double v = tN[N * (i * OD + ao) + j * OD + bo];
+                if (labA[i]) atomicAdd(&a.val[pos * OO + ao * OD + bo], -v);
+                if (labBj) atomicAdd(&a.val[(long long)a.tpos[pos] * OO + bo * OD + ao], -v);
+            }
+        if (labBj) {  // +termLocalPrime[N (i OD+ao) + j OD+bo] into (bAdx[j], bAdx[i])
+            const long long q = a.posTT[9 * b + 3 * j + i];
+#pragma unroll
+            for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                for (int bo = 0; bo < OD; bo++)
+                    atomicAdd(&a.val[q * OO + bo * OD + ao], tLP[N * (i * OD + ao) + j * OD + bo]);
+        }
+    }
+}
+
+// termLocal of outer element aT: per-thread sums reduced over the 256-thread block, then
+// added into (aAdx[i], aAdx[j]).
+template <int OD>
+NLFEM_DEV void cut_flush_local(const NumArgs& a, int aT, const double* tL, const bool* labA, double (*red)[9 * OD * OD]) {
+    constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        double v = tL[i];
+        for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+        if (lane == 0) red[w][i] = v;
+    }
+    __syncthreads();
+    for (int t = tid; t < NB; t += 256) {
+        const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        const int ra = t / N, cb = t % N;
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (!labA[i]) continue;
+        const long long p = a.posTT[9 * aT + 3 * i + j];
+        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
     }
 }
 
@@ -1203,6 +1290,64 @@ constexpr int CUT_ITEMS = 128;
 constexpr int CUT_MAXP = 9;
 constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= CUT_ITEMS / CUT_MAXPAIRS = 4
 
+// Surrogate item (outer point x, inner element E): ZERO when B(x, delta) misses E, FULL when
+// it contains E, PARTIAL otherwise; OUT when the item lies outside the network's domain
+// (x inside E, or E's size relative to delta outside [smin, smax]).  For PARTIAL items f
+// receives the 12 network inputs: E's vertices relative to x divided by delta, rotated so
+// that E's centroid lies on the +x axis (6), (|u_i|^2 - 1) / s (3) and (dist(0, edge) - 1) / s
+// for the edges 01, 12, 20 (3), with s = sqrt|det| in delta units.
+enum SurItem : int { SUR_ZERO = 0, SUR_FULL = 1, SUR_PARTIAL = 2, SUR_OUT = 3 };
+
+NLFEM_DEV int sur_item(double x0, double x1, const double* E, double sqdelta, double idelta, double smin,
+                       double smax, float* f) {
+    double u[6], r2[3];
+    int nin = 0;
+    for (int i = 0; i < 3; i++) {
+        const double d0 = E[2 * i] - x0, d1 = E[2 * i + 1] - x1;
+        r2[i] = d0 * d0 + d1 * d1;
+        nin += r2[i] <= sqdelta;
+        u[2 * i] = d0 * idelta;
+        u[2 * i + 1] = d1 * idelta;
+    }
+    if (nin == 3) return SUR_FULL;
+    double ed[3];
+    double dmin = 1e300;
+    for (int e = 0; e < 3; e++) {
+        const int i = e, j = (e + 1) % 3;
+        const double P0 = u[2 * i], P1 = u[2 * i + 1], D0 = u[2 * j] - P0, D1 = u[2 * j + 1] - P1;
+        const double dd = D0 * D0 + D1 * D1;
+        double t = dd > 0 ? -(P0 * D0 + P1 * D1) / dd : 0.0;
+        t = t < 0 ? 0 : (t > 1 ? 1 : t);
+        const double C0 = P0 + t * D0, C1 = P1 + t * D1;
+        ed[e] = sqrt(C0 * C0 + C1 * C1);
+        dmin = ed[e] < dmin ? ed[e] : dmin;
+    }
+    // is x inside E?  (signs of the three edge cross products relative to the origin)
+    const double c0 = u[0] * u[3] - u[1] * u[2], c1 = u[2] * u[5] - u[3] * u[4], c2 = u[4] * u[1] - u[5] * u[0];
+    const bool inside = (c0 >= 0 && c1 >= 0 && c2 >= 0) || (c0 <= 0 && c1 <= 0 && c2 <= 0);
+    if (inside) return SUR_OUT;
+    if (nin == 0 && dmin > 1.0) return SUR_ZERO;
+    const double det = fabs((u[2] - u[0]) * (u[5] - u[1]) - (u[4] - u[0]) * (u[3] - u[1]));
+    const double sz = sqrt(det);
+    if (!(sz >= smin && sz <= smax)) return SUR_OUT;
+    const double g0 = (u[0] + u[2] + u[4]) / 3.0, g1 = (u[1] + u[3] + u[5]) / 3.0;
+    const double gn = sqrt(g0 * g0 + g1 * g1);
+    const double cs = g0 / gn, sn = g1 / gn, is = 1.0 / sz;
+    for (int i = 0; i < 3; i++) {
+        f[2 * i] = (float)(cs * u[2 * i] + sn * u[2 * i + 1]);
+        f[2 * i + 1] = (float)(-sn * u[2 * i] + cs * u[2 * i + 1]);
+        f[6 + i] = (float)((u[2 * i] * u[2 * i] + u[2 * i + 1] * u[2 * i + 1] - 1.0) * is);
+        f[9 + i] = (float)((ed[i] - 1.0) * is);
+    }
+    return SUR_PARTIAL;
+}
+
+NLFEM_DEV unsigned sur_hash(unsigned aT, unsigned b, unsigned k, unsigned salt) {
+    unsigned h = aT * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (k + 1u) * 0xC2B2AE3Du ^ salt;
+    h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+    return h;
+}
+
 struct PointCollector {
     double2* pts;
     int nt;
@@ -1218,7 +1363,9 @@ struct PointCollector {
     }
 };
 
-template <int KER, int OD, typename T>
+// MODE 0 assembles; MODE 1 writes training samples of the surrogate (partial items, exact
+// FP64 inner sums as targets) and assembles nothing.
+template <int KER, int OD, typename T, int MODE = 0>
 __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     extern __shared__ double ldsc[];
@@ -1239,7 +1386,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     const int lane = tid & 63, w = tid >> 6;
     const ElemRec A = a.elem[aT];
     if (A.label == 0) return;
-    const int nB = a.cntB[aT];
+    const int nB = a.sur.list_in ? a.sur.rcnt[aT] : a.cntB[aT];
     if (nB == 0) return;
     const int nPx = a.q.nPx;
     const int CP = min(CUT_ITEMS / nPx, CUT_MAXPAIRS);
@@ -1272,8 +1419,8 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         __syncthreads();
         for (int i = tid; i < CUT_ITEMS * 10 * OO; i += 256) iacc[i] = 0.0;
         if (tid < np) {
-            const uint32_t e = a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)];
-            const int b = (int)(e & IDX_MASK);
+            const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + c0 + tid]
+                                        : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)] & IDX_MASK);
             pb_[tid] = b;
             lus[tid] = pivot_lu(a.elem[b].E);
         }
@@ -1350,6 +1497,31 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             for (int o = 0; o < 6 * OO; o++) atomicAdd(&ia[4 * OO + o], sLP[o]);
         }
         __syncthreads();
+        if constexpr (MODE == 1) {  // training samples: ratios of the exact sums to the full-triangle sums
+            if (tid < nit && OD == 1) {
+                const int p = tid / nPx, k = tid % nPx;
+                const int b = pb_[p];
+                if (sur_hash((unsigned)aT, (unsigned)b, (unsigned)k, a.sur.salt) % a.sur.stride == 0) {
+                    const ElemRec B = a.elem[b];
+                    double x[2];
+                    toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+                    float f[12];
+                    if (sur_item(x[0], x[1], B.E, a.kp.sqdelta, 1.0 / sqrt(a.kp.sqdelta), a.sur.smin, a.sur.smax, f) ==
+                        SUR_PARTIAL) {
+                        const long long idx = (long long)atomicAdd(&a.counters[CNT_SAMPLES], 1ull);
+                        if (idx < a.sur.smaxn) {
+                            const double* ia = iacc + tid * 10;
+                            const double cd = a.kp.c * B.absDet;
+                            for (int i = 0; i < 12; i++) a.sur.sfeat[idx * 12 + i] = f[i];
+                            for (int q = 0; q < 6; q++) a.sur.starg[idx * 9 + q] = (float)(ia[4 + q] / (cd * a.sur.q2[q]));
+                            for (int bb = 0; bb < 3; bb++)
+                                a.sur.starg[idx * 9 + 6 + bb] = (float)(ia[1 + bb] / (cd * a.sur.q1[bb]));
+                        }
+                    }
+                }
+            }
+            continue;
+        }
         // C: outer accumulation (src/integration.cpp:508-537), one thread per pair
         if (tid < np) {
             double tN[NB], tLP[NB];
@@ -1366,53 +1538,218 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
         }
         __syncthreads();
-        // D: scatter (src/Cassemble.cpp:721-770): thread per (pair, i, j)
-        for (int t = tid; t < np * 9; t += 256) {
-            const int p = t / 9, i = (t % 9) / 3, j = t % 3;
-            const int b = pb_[p];
-            const double* tN = pacc + p * 2 * NB;
-            const double* tLP = tN + NB;
-            const int rBj = node_of(a.tri, b, j, a.is_DG);
-            const bool labBj = row_labelled(a.vlabel, rBj, a.is_DG);
-            const int s = lds_find(hk, a.HN, rBj);
-            if (s < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
-            const long long pos = a.rowptr[rA[i]] + hv[3 * s + i];  // entry (aAdx[i], bAdx[j])
-#pragma unroll
-            for (int ao = 0; ao < OD; ao++)
-#pragma unroll
-                for (int bo = 0; bo < OD; bo++) {
-                    const double v = tN[N * (i * OD + ao) + j * OD + bo];
-                    if (labA[i]) atomicAdd(&a.val[pos * OO + ao * OD + bo], -v);
-                    if (labBj) atomicAdd(&a.val[(long long)a.tpos[pos] * OO + bo * OD + ao], -v);
-                }
-            if (labBj) {  // +termLocalPrime[N (i OD+ao) + j OD+bo] into (bAdx[j], bAdx[i])
-                const long long q = a.posTT[9 * b + 3 * j + i];
-#pragma unroll
-                for (int ao = 0; ao < OD; ao++)
-#pragma unroll
-                    for (int bo = 0; bo < OD; bo++)
-                        atomicAdd(&a.val[q * OO + bo * OD + ao], tLP[N * (i * OD + ao) + j * OD + bo]);
-            }
-        }
+        // D: scatter (src/Cassemble.cpp:721-770)
+        cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
     }
-    // termLocal: block reduction, then into (aAdx[i], aAdx[j])
+    if constexpr (MODE == 1) return;
     atomicAdd(&subtri, mytri);
-#pragma unroll
-    for (int i = 0; i < NB; i++) {
-        double v = tL[i];
-        for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
-        if (lane == 0) red[w][i] = v;
-    }
     __syncthreads();
     if (tid == 0) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)subtri);
-    for (int t = tid; t < NB; t += 256) {
-        const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-        const int ra = t / N, cb = t % N;
-        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
-        if (!labA[i]) continue;
-        const long long p = a.posTT[9 * aT + 3 * i + j];
-        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    // termLocal: block reduction, then into (aAdx[i], aAdx[j])
+    cut_flush_local<OD>(a, aT, tL, labA, red);
+}
+
+// Cut pairs through the surrogate (constant kernel, scalar), one 256-thread block per outer
+// element, chunks of up to SUR_MAXPAIRS pairs:
+//   A  thread per (pair, outer point) item: ZERO / FULL / PARTIAL / OUT (sur_item); the
+//      features of PARTIAL items are compacted into LDS
+//   B  each wavefront runs 16-item tiles through the network(s) with BF16 MFMA
+//   C  thread per pair: a pair with an OUT item or an item whose ensemble spread exceeds
+//      tol goes to the exact method (rlist); otherwise its inner sums are rebuilt from the
+//      full-triangle moments and the predicted ratios (M := row sums of S) and accumulated
+//      over the outer points exactly as in the exact cut kernel
+//   D  scatter of the accepted pairs (cut_scatter), termLocal at the end (cut_flush_local).
+constexpr int SUR_ITEMS = 256;
+constexpr int SUR_MAXPAIRS = 36;
+
+template <int KER>
+__global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
+    constexpr int OD = 1, NB = 9;
+    extern __shared__ int ldss[];
+    int* hk = ldss;       // [HN]
+    int* hv = hk + a.HN;  // [HN][3]
+    __shared__ float feat[SUR_ITEMS][12];
+    __shared__ float outp[SUR_ITEMS][9];
+    __shared__ float spread[SUR_ITEMS][4];
+    __shared__ unsigned char icls[SUR_ITEMS];
+    __shared__ short islot[SUR_ITEMS];
+    __shared__ int pb_[SUR_MAXPAIRS];
+    __shared__ int prej[SUR_MAXPAIRS];
+    __shared__ double pacc[SUR_MAXPAIRS * 2 * NB];
+    __shared__ double red[4][NB];
+    __shared__ int npart, nrej, nacc;
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nB = a.cntB[aT];
+    if (nB == 0) {
+        if (tid == 0) a.sur.rcnt[aT] = 0;
+        return;
     }
+    const int nPx = a.q.nPx;
+    const int CP = min(SUR_ITEMS / nPx, SUR_MAXPAIRS);
+    const double idelta = 1.0 / sqrt(a.kp.sqdelta);
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    if (tid == 0) { nrej = 0; nacc = 0; }
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256) {
+            int s = lds_slot(hk, a.HN, a.col[r0 + i]);
+            if (s >= 0) hv[3 * s + j] = i;
+            else atomicOr(&a.counters[CNT_ERROR], 16ull);
+        }
+    }
+    double tL[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) tL[i] = 0;
+    const double absA = A.absDet;
+    unsigned myitems = 0;
+    for (int c0 = 0; c0 < nB; c0 += CP) {
+        const int np = min(CP, nB - c0);
+        const int nit = np * nPx;
+        __syncthreads();
+        if (tid == 0) npart = 0;
+        if (tid < np) {
+            pb_[tid] = (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)] & IDX_MASK);
+            prej[tid] = 0;
+        }
+        __syncthreads();
+        // A: classify the items, compact the features of the partial ones
+        if (tid < nit) {
+            const int p = tid / nPx, k = tid % nPx;
+            const ElemRec B = a.elem[pb_[p]];
+            double x[2];
+            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+            float f[12];
+            const int c = sur_item(x[0], x[1], B.E, a.kp.sqdelta, idelta, a.sur.smin, a.sur.smax, f);
+            icls[tid] = (unsigned char)c;
+            if (c == SUR_PARTIAL) {
+                const int sl = atomicAdd(&npart, 1);
+                islot[tid] = (short)sl;
+#pragma unroll
+                for (int i = 0; i < 12; i++) feat[sl][i] = f[i];
+            } else if (c == SUR_OUT) {
+                prej[p] = 1;
+            }
+        }
+        __syncthreads();
+        // B: network tiles of 16 items per wavefront
+        const int npi = npart;
+        myitems += (tid == 0) ? npi : 0;
+        {
+            const int j = lane & 15, q = 4 * (lane >> 4);
+            for (int tile = w; tile * 16 < npi; tile += 4) {
+                const int sl = tile * 16 + j;
+                const bool valid = sl < npi;
+                float o0[4] = {0, 0, 0, 0}, o1[4] = {0, 0, 0, 0};
+                for (int net = 0; net < a.sur.nnet; net++) {
+                    const float* nm = a.sur.norm + net * mlp::NNORM;
+                    mlp::f32x4 xin = {0, 0, 0, 0};
+                    if (valid && q < mlp::NIN) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) xin[i] = (feat[sl][q + i] - nm[mlp::NORM_MU + q + i]) * nm[mlp::NORM_ISD + q + i];
+                    }
+                    const mlp::f32x4 y = mlp::forward_tile(a.sur.wf + (size_t)net * mlp::NFRAG_FWD * 64,
+                                                           a.sur.bias + net * mlp::BIAS_PAD, xin, lane);
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int o = q + i < mlp::NOUT ? q + i : 0;
+                        const float v = y[i] * nm[mlp::NORM_OSD + o] + nm[mlp::NORM_OMU + o];
+                        if (net == 0) o0[i] = v;
+                        else o1[i] = v;
+                    }
+                }
+                if (valid) {
+                    float sp = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        if (q + i >= mlp::NOUT) continue;
+                        const float v = a.sur.nnet > 1 ? 0.5f * (o0[i] + o1[i]) : o0[i];
+                        outp[sl][q + i] = v;
+                        if (a.sur.nnet > 1) sp = fmaxf(sp, fabsf(o0[i] - o1[i]));
+                    }
+                    spread[sl][lane >> 4] = sp;
+                }
+            }
+        }
+        __syncthreads();
+        // C: acceptance and outer accumulation, one thread per pair
+        if (tid < np) {
+            const int p = tid;
+            bool rej = prej[p] != 0;
+            for (int k = 0; k < nPx && !rej; k++) {
+                const int it = p * nPx + k;
+                if (icls[it] == SUR_PARTIAL) {
+                    const int sl = islot[it];
+                    const float sp = fmaxf(fmaxf(spread[sl][0], spread[sl][1]), fmaxf(spread[sl][2], spread[sl][3]));
+                    if (!(sp <= a.sur.tol)) rej = true;
+                }
+            }
+            const int b = pb_[p];
+            if (rej) {
+                const int r = atomicAdd(&nrej, 1);
+                a.sur.rlist[a.sur.roff[aT] + r] = b;
+                pb_[p] = -1;
+            } else {
+                atomicAdd(&nacc, 1);
+                const double cd = a.kp.c * a.elem[b].absDet;
+                double tN[NB], tLP[NB];
+#pragma unroll
+                for (int o = 0; o < NB; o++) { tN[o] = 0; tLP[o] = 0; }
+                for (int k = 0; k < nPx; k++) {
+                    const int it = p * nPx + k;
+                    const int c = icls[it];
+                    if (c == SUR_ZERO) continue;
+                    double inL, inN[3], sLP[6];
+                    if (c == SUR_FULL) {
+                        inL = cd * a.sur.q0;
+#pragma unroll
+                        for (int bb = 0; bb < 3; bb++) inN[bb] = cd * a.sur.q1[bb];
+#pragma unroll
+                        for (int qq = 0; qq < 6; qq++) sLP[qq] = cd * a.sur.q2[qq];
+                    } else {
+                        const float* y = outp[islot[it]];
+#pragma unroll
+                        for (int qq = 0; qq < 6; qq++) {
+                            const float r = fminf(fmaxf(y[qq], 0.0f), 1.0f);
+                            sLP[qq] = cd * a.sur.q2[qq] * (double)r;
+                        }
+                        // M_a := sum_b S_ab keeps the partition of unity of the pair exact
+                        inN[0] = sLP[0] + sLP[1] + sLP[2];
+                        inN[1] = sLP[1] + sLP[3] + sLP[4];
+                        inN[2] = sLP[2] + sLP[4] + sLP[5];
+                        inL = inN[0] + inN[1] + inN[2];
+                    }
+                    double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+                    outer_point<OD>(psx, absA * a.q.dx[k], &inL, inN, tL, tN);
+                    add_lp<OD>(sLP, a.q.dx[k] * absA, tLP);
+                }
+                double* pa = pacc + p * 2 * NB;
+#pragma unroll
+                for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
+            }
+        }
+        __syncthreads();
+        cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        a.sur.rcnt[aT] = nrej;
+        atomicAdd(&a.counters[CNT_SUR_ACC], (unsigned long long)nacc);
+        atomicAdd(&a.counters[CNT_SUR_REJ], (unsigned long long)nrej);
+        atomicAdd(&a.counters[CNT_SUR_ITEMS], (unsigned long long)myitems);
+    }
+    cut_flush_local<OD>(a, aT, tL, labA, red);
 }
 
 // ======================================================================= close pairs
@@ -1774,6 +2111,57 @@ void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* ver
 
 static long long* scan_scratch(Plan& P) { return (long long*)P.d_scan_tmp; }
 
+namespace mlp {
+void upload(const float* params, const float* norm, uint16_t* d_wf, float* d_bias, float* d_norm, hipStream_t st);
+}
+
+static NumArgs base_numargs(const Plan& P) {
+    const PlanConfig& c = P.cfg;
+    NumArgs na{};
+    na.nE = c.nE; na.cap = P.cap; na.is_DG = c.is_DG; na.caps = c.caps; na.od = c.outdim;
+    na.elem = P.d_elem; na.tri = P.d_tri; na.vlabel = P.d_vlabel; na.nbr = P.d_nbr; na.cntF = P.d_cntF;
+    na.cntB = P.d_cntB; na.rowptr = P.d_rowptr; na.col = P.d_col; na.tpos = P.d_tpos; na.posTT = P.d_posTT;
+    na.val = P.d_val; na.q = c.q; na.kp = kernel_params(c); na.counters = P.d_counters;
+    na.sur = SurArgs{};
+    na.sur.stride = 1;
+    return na;
+}
+
+// The surrogate covers cut pairs of the scalar constant kernel with the retriangulate method
+// and the 7-point outer rule it was trained with.
+static bool surrogate_applies(const Plan& P) {
+    const PlanConfig& c = P.cfg;
+    return P.sur.nnet > 0 && c.kernel == KER_CONSTANT && c.outdim == 1 && c.method_remote == M_RETRIANGULATE &&
+           !c.singular && P.stats.bary == 0 && c.q.nPx <= SUR_ITEMS / 4;
+}
+
+static void set_surrogate_args(const Plan& P, SurArgs& s) {
+    const QuadArgs& q = P.cfg.q;
+    s.wf = (const mlp::bf16x4*)P.sur.d_wf;
+    s.bias = P.sur.d_bias;
+    s.norm = P.sur.d_norm;
+    s.nnet = P.sur.nnet;
+    s.tol = P.sur.tol;
+    s.smin = P.sur.smin;
+    s.smax = P.sur.smax;
+    // full-triangle sums of the inner rule in the order of inner_point
+    s.q0 = 0;
+    for (int i = 0; i < 3; i++) s.q1[i] = 0;
+    for (int i = 0; i < 6; i++) s.q2[i] = 0;
+    for (int i = 0; i < q.nPy; i++) {
+        const double* ps = &q.psiy[3 * i];
+        const double pp[6] = {ps[0] * ps[0], ps[0] * ps[1], ps[0] * ps[2], ps[1] * ps[1], ps[1] * ps[2], ps[2] * ps[2]};
+        s.q0 += q.dy[i];
+        for (int b = 0; b < 3; b++) s.q1[b] += ps[b] * q.dy[i];
+        for (int k = 0; k < 6; k++) s.q2[k] += pp[k] * q.dy[i];
+    }
+    s.rlist = P.d_rlist;
+    s.roff = P.d_roff;
+    s.rcnt = P.d_rcnt;
+    s.list_in = 0;
+    s.stride = 1;
+}
+
 void plan_assemble(Plan& P, const Estimates& est) {
     const PlanConfig& c = P.cfg;
     const hipStream_t st = P.stream;
@@ -1816,7 +2204,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- BFS
     const double delta = std::sqrt(c.sqdelta);
-    double area_est = M_PI * std::pow(delta + 2.0 * est.hmax, 2) / est.amin;
+    double area_est = M_PI * std::pow(delta + 2.0 * est.hmax, 2) / est.aref();
     long long visit = c.fullCC ? (long long)c.nE : (long long)(1.3 * area_est) + 128;
     if (visit > c.nE) visit = c.nE;
     dalloc(P.d_cntF, (size_t)c.nE);
@@ -1903,7 +2291,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- pattern
     ev_begin(P, PH_PATTERN);
-    const double pat_est = M_PI * std::pow(delta + 3.0 * est.hmax, 2) / est.amin;
+    const double pat_est = M_PI * std::pow(delta + 3.0 * est.hmax, 2) / est.aref();
     long long ecap = c.fullCC ? c.nE : std::min<long long>((long long)(1.4 * pat_est) + 256, c.nE);
     PatArgs pa;
     pa.nNodes = P.nNodes; pa.is_DG = c.is_DG; pa.cap = P.cap; pa.tri = P.d_tri; pa.vptr = P.d_vptr;
@@ -1964,11 +2352,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     (void)hipFree(d_rowOf);
 
     // ---------------------------------------------------------------- quadrature
-    NumArgs na;
-    na.nE = c.nE; na.cap = P.cap; na.is_DG = c.is_DG; na.caps = c.caps; na.od = c.outdim;
-    na.elem = P.d_elem; na.tri = P.d_tri; na.vlabel = P.d_vlabel; na.nbr = P.d_nbr; na.cntF = P.d_cntF;
-    na.cntB = P.d_cntB; na.rowptr = P.d_rowptr; na.col = P.d_col; na.tpos = P.d_tpos; na.posTT = P.d_posTT;
-    na.val = P.d_val; na.q = c.q; na.kp = kernel_params(c); na.counters = P.d_counters;
+    NumArgs na = base_numargs(P);
     // LDS map of the three rows of the outer element: 16 bytes per slot
     long long HN = next_pow2((long long)(2.5 * std::max(P.maxRow, 1)));
     na.HN = (HN * 16 <= 64 * 1024) ? (int)HN : 0;
@@ -2068,10 +2452,34 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                          \
     } while (0)
     ev_begin(P, PH_CUT);
+    P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
+    const bool use_sur = surrogate_applies(P);
     if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
         DISPATCH_PAIRS(1);
     } else {
         NumArgs nc = na;
+        if (use_sur) {
+            // surrogate first; the pairs it hands back go through the exact kernel below
+            dalloc(P.d_roff, (size_t)c.nE + 1);
+            dalloc(P.d_rcnt, (size_t)c.nE);
+            scan_exclusive<int>(P.d_cntB, P.d_roff, c.nE, scan_scratch(P), st);
+            long long ncut = 0;
+            HIPCHECK(hipMemcpyAsync(&ncut, P.d_roff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            if (P.rlist_cap < ncut || !P.d_rlist) {
+                dalloc(P.d_rlist, (size_t)std::max<long long>(ncut, 1));
+                P.rlist_cap = std::max<long long>(ncut, 1);
+            }
+            NumArgs ns = na;
+            ns.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+            set_surrogate_args(P, ns.sur);
+            const size_t ls = (size_t)ns.HN * 16;
+            if (ls > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur<KER_CONSTANT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+            k_cut_sur<KER_CONSTANT><<<c.nE, 256, ls, st>>>(ns);
+            HIPCHECK(hipGetLastError());
+            nc.sur = ns.sur;
+            nc.sur.list_in = 1;
+        }
         nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
         const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP +
                           sizeof(double) * (CUT_ITEMS * 10 * P.oo + CUT_MAXPAIRS * 18 * P.oo) + (size_t)nc.HN * 16;
@@ -2125,6 +2533,9 @@ void plan_assemble(Plan& P, const Estimates& est) {
     P.stats.nnz_nodes = P.nnz_nodes;
     P.stats.nnz = P.nnz;
     P.stats.subtri = h[CNT_SUBTRI];
+    P.stats.sur_acc = h[CNT_SUR_ACC];
+    P.stats.sur_rej = h[CNT_SUR_REJ];
+    P.stats.sur_items = h[CNT_SUR_ITEMS];
     if (P.timing) {
         for (int ph = PH_DUAL; ph < PH_COUNT; ph++) {
             float ms = 0;
@@ -2136,7 +2547,86 @@ void plan_assemble(Plan& P, const Estimates& est) {
     P.assembled = true;
 }
 
+void plan_set_surrogate(Plan& P, int nnet, const float* params, const float* norm, float tol, double smin,
+                        double smax) {
+    if (nnet < 0 || nnet > 2) throw std::invalid_argument("the surrogate ensemble has one or two networks");
+    if (nnet > 0 && (!params || !norm)) throw std::invalid_argument("surrogate parameters missing");
+    if (nnet > 0 && !(tol >= 0)) throw std::invalid_argument("surrogate tolerance must be >= 0");
+    if (nnet > 0 && !(smin >= 0 && smax > smin)) throw std::invalid_argument("invalid surrogate domain");
+    if (!P.sur.d_wf) {
+        HIPCHECK(hipMalloc((void**)&P.sur.d_wf, 2 * mlp::NFRAG_FWD * 256 * sizeof(uint16_t)));
+        HIPCHECK(hipMalloc((void**)&P.sur.d_bias, 2 * mlp::BIAS_PAD * sizeof(float)));
+        HIPCHECK(hipMalloc((void**)&P.sur.d_norm, 2 * mlp::NNORM * sizeof(float)));
+    }
+    for (int n = 0; n < nnet; n++)
+        mlp::upload(params + (size_t)n * mlp::NPARAM, norm + (size_t)n * mlp::NNORM,
+                    P.sur.d_wf + (size_t)n * mlp::NFRAG_FWD * 256, P.sur.d_bias + n * mlp::BIAS_PAD,
+                    P.sur.d_norm + n * mlp::NNORM, P.stream);
+    HIPCHECK(hipStreamSynchronize(P.stream));
+    P.sur.nnet = nnet;
+    P.sur.tol = tol;
+    P.sur.smin = smin;
+    P.sur.smax = smax;
+}
+
+// Training samples of the surrogate from the exact FP64 cut-pair sums of the assembled plan:
+// partial items whose hash(aT, bT, k, salt) is a multiple of stride.  Returns the number of
+// qualifying items (those beyond maxn are not stored).
+long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsigned salt, double smin, double smax,
+                                 float* feat, float* targ) {
+    const PlanConfig& c = P.cfg;
+    if (!P.assembled) throw std::invalid_argument("assemble the plan before drawing surrogate samples");
+    if (c.kernel != KER_CONSTANT || c.outdim != 1 || c.method_remote != M_RETRIANGULATE || P.stats.bary > 0)
+        throw std::invalid_argument("surrogate samples need the constant kernel with the retriangulate method");
+    if (c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS)
+        throw std::invalid_argument("outer rule size not supported by the cut kernel");
+    if (stride == 0 || maxn < 0 || (maxn > 0 && (!feat || !targ)) || !(smax > smin))
+        throw std::invalid_argument("invalid sample arguments");
+    const hipStream_t st = P.stream;
+    float *dF = nullptr, *dT = nullptr;
+    HIPCHECK(hipMalloc((void**)&dF, (size_t)std::max<long long>(maxn, 1) * 12 * sizeof(float)));
+    HIPCHECK(hipMalloc((void**)&dT, (size_t)std::max<long long>(maxn, 1) * 9 * sizeof(float)));
+    NumArgs nc = base_numargs(P);
+    nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+    nc.sur.sfeat = dF;
+    nc.sur.starg = dT;
+    nc.sur.smaxn = maxn;
+    nc.sur.stride = stride;
+    nc.sur.salt = salt;
+    nc.sur.smin = smin;
+    nc.sur.smax = smax;
+    set_surrogate_args(P, nc.sur);
+    nc.sur.sfeat = dF; nc.sur.starg = dT; nc.sur.smaxn = maxn; nc.sur.stride = stride; nc.sur.salt = salt;
+    nc.sur.smin = smin; nc.sur.smax = smax;
+    const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP + sizeof(double) * (CUT_ITEMS * 10 + CUT_MAXPAIRS * 18) +
+                      (size_t)nc.HN * 16;
+    unsigned long long h[16];
+    try {
+        HIPCHECK(hipMemsetAsync(P.d_counters + CNT_SAMPLES, 0, sizeof(unsigned long long), st));
+        if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER_CONSTANT, 1, double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+        k_cut<KER_CONSTANT, 1, double, 1><<<c.nE, 256, lb, st>>>(nc);
+        HIPCHECK(hipGetLastError());
+        read_counters(P, h);
+        const long long n = (long long)h[CNT_SAMPLES];
+        const long long m = std::min(n, maxn);
+        if (m > 0) {
+            HIPCHECK(hipMemcpy(feat, dF, (size_t)m * 12 * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(targ, dT, (size_t)m * 9 * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        (void)hipFree(dF);
+        (void)hipFree(dT);
+        return n;
+    } catch (...) {
+        (void)hipFree(dF);
+        (void)hipFree(dT);
+        throw;
+    }
+}
+
 void plan_free(Plan& P) {
+    void* sp[] = {P.sur.d_wf, P.sur.d_bias, P.sur.d_norm, P.d_rlist, P.d_roff, P.d_rcnt};
+    for (void* p : sp)
+        if (p) (void)hipFree(p);
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,

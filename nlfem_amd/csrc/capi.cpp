@@ -16,6 +16,10 @@ namespace nlfem {
 void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* verts, const long* vlab64);
 void plan_assemble(Plan& P, const Estimates& est);
 void plan_free(Plan& P);
+void plan_set_surrogate(Plan& P, int nnet, const float* params, const float* norm, float tol, double smin,
+                        double smax);
+long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsigned salt, double smin, double smax,
+                                 float* feat, float* targ);
 void run_forcing(int K, const long* tri, const long* lab, const double* verts, const long* vlab, int nE, int nV,
                  const double* Px, int nPx, const double* dx, int is_DG, int od, int f, double* fd_host);
 void run_mass(int K, const long* tri, const long* lab, const double* verts, const long* vlab, int J, int nE, int nV,
@@ -156,7 +160,7 @@ static nlfem_plan* make_plan(int K, const long* tri, const long* lab, const doub
         throw;
     }
     // geometric estimates for the BFS / pattern capacities
-    double hmax = 0, amin = 1e300;
+    double hmax = 0, amin = 1e300, asum = 0;
     for (int e = 0; e < nE; e++) {
         const long* t = tri + 3 * (size_t)e;
         double E[6];
@@ -172,9 +176,11 @@ static nlfem_plan* make_plan(int K, const long* tri, const long* lab, const doub
         double a = 0.5 * std::fabs((E[2] - E[0]) * (E[5] - E[1]) - (E[4] - E[0]) * (E[3] - E[1]));
         if (!(a > 0)) { delete h; throw std::invalid_argument("degenerate element (zero area)"); }
         amin = std::min(amin, a);
+        asum += a;
     }
     h->est.hmax = hmax;
     h->est.amin = amin;
+    h->est.amean = nE > 0 ? asum / nE : 1.0;
     try {
         plan_setup(h->P, tri, lab, verts, vlab);
     } catch (...) {
@@ -279,14 +285,35 @@ extern "C" int nlfem_plan_neighbours(const nlfem_plan* plan, long long* ptr, int
 extern "C" int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n) {
     if (!plan) return fail(NLFEM_EINVAL, "null plan");
     const Stats& s = plan->P.stats;
-    double v[11 + PH_COUNT];
+    double v[14 + PH_COUNT];
     v[0] = (double)s.pairs; v[1] = (double)s.probes; v[2] = (double)s.close; v[3] = (double)s.interior;
     v[4] = (double)s.cut; v[5] = (double)s.bary; v[6] = (double)s.nnz_nodes; v[7] = (double)s.nnz;
     v[8] = (double)s.cap; v[9] = (double)s.max_list; v[10] = (double)s.subtri;
     for (int ph = PH_DUAL; ph < PH_COUNT; ph++) v[11 + ph - PH_DUAL] = s.ms[ph];
-    const int nv = 11 + PH_COUNT - PH_DUAL;
+    int nv = 11 + PH_COUNT - PH_DUAL;
+    v[nv++] = (double)s.sur_acc;
+    v[nv++] = (double)s.sur_rej;
+    v[nv++] = (double)s.sur_items;
     for (int i = 0; i < n && i < nv; i++) out[i] = v[i];
     return NLFEM_OK;
+}
+
+extern "C" int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm,
+                                        float tol, double smin, double smax) {
+    if (!plan) return fail(NLFEM_EINVAL, "null plan");
+    GUARD_BEGIN
+    plan_set_surrogate(plan->P, nnet, params, norm, tol, smin, smax);
+    return NLFEM_OK;
+    GUARD_END
+}
+
+extern "C" int nlfem_plan_surrogate_samples(nlfem_plan* plan, long long maxn, unsigned stride, unsigned salt,
+                                            double smin, double smax, float* feat, float* targ, long long* count) {
+    if (!plan || !count) return fail(NLFEM_EINVAL, "null argument");
+    GUARD_BEGIN
+    *count = plan_surrogate_samples(plan->P, maxn, stride, salt, smin, smax, feat, targ);
+    return NLFEM_OK;
+    GUARD_END
 }
 
 extern "C" void nlfem_plan_destroy(nlfem_plan* plan) {

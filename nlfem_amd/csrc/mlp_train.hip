@@ -306,6 +306,30 @@ static void pack(DevNet& d, hipStream_t st) {
     HIPCHECK_MLP(hipGetLastError());
 }
 
+// Pack one network into caller-owned device buffers (fragments NFRAG_FWD*256 bf16, biases
+// BIAS_PAD, normalisation NNORM) on stream st.
+void upload(const float* params, const float* norm, uint16_t* d_wf, float* d_bias, float* d_norm, hipStream_t st) {
+    check_norm(norm);
+    float* w = dmalloc<float>(NPARAM);
+    uint16_t* wb = nullptr;
+    try {
+        wb = dmalloc<uint16_t>(NFRAG_BWD * 256);
+        HIPCHECK_MLP(hipMemcpyAsync(w, params, NPARAM * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK_MLP(hipMemcpyAsync(d_norm, norm, NNORM * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK_MLP(hipMemsetAsync(d_wf, 0, NFRAG_FWD * 256 * sizeof(uint16_t), st));
+        HIPCHECK_MLP(hipMemsetAsync(d_bias, 0, BIAS_PAD * sizeof(float), st));
+        k_mlp_update<<<(NPARAM + 255) / 256, 256, 0, st>>>(0, nullptr, w, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f,
+                                                           1.f, 1.f, d_wf, wb, d_bias);
+        HIPCHECK_MLP(hipGetLastError());
+        HIPCHECK_MLP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipFree(w); (void)hipFree(wb);
+        throw;
+    }
+    (void)hipFree(w);
+    (void)hipFree(wb);
+}
+
 void mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y) {
     check_norm(norm);
     if (n <= 0) return;

@@ -38,6 +38,7 @@ enum Phase : int {
 
 struct Stats {
     unsigned long long pairs = 0, probes = 0, close = 0, interior = 0, cut = 0, bary = 0, subtri = 0;
+    unsigned long long sur_acc = 0, sur_rej = 0, sur_items = 0;  // cut pairs by the surrogate / handed back; network items
     long long nnz_nodes = 0, nnz = 0;
     int cap = 0, max_list = 0;
     float ms[PH_COUNT] = {0};
@@ -45,11 +46,30 @@ struct Stats {
 
 // Geometric size estimates for the LDS capacities (grown and retried on overflow).
 struct Estimates {
-    double hmax = 0, amin = 1e300;
+    double hmax = 0, amin = 1e300, amean = 1.0;
+    // element area for counting elements in a disc: the mean, unless the mesh is nearly
+    // uniform below it (capacities are grown and the kernels rerun on overflow)
+    double aref() const { return amin > 0.6 * amean ? amin : 0.6 * amean; }
+};
+
+// Surrogate networks for cut pairs (constant kernel): packed BF16 fragments, biases and
+// normalisation per network, acceptance threshold and domain.
+struct SurrogateModel {
+    int nnet = 0;
+    float tol = 0;
+    double smin = 0, smax = 0;
+    uint16_t* d_wf = nullptr;
+    float* d_bias = nullptr;
+    float* d_norm = nullptr;
 };
 
 struct Plan {
     PlanConfig cfg;
+    SurrogateModel sur;
+    int* d_rlist = nullptr;       // cut pairs handed back to the exact method
+    long long* d_roff = nullptr;  // [nE+1]
+    int* d_rcnt = nullptr;        // [nE]
+    long long rlist_cap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2 * PH_COUNT] = {};
     bool timing = true;

@@ -23,7 +23,7 @@ import time
 import numpy as np
 import scipy.sparse as sp
 
-from ._lib import lib, check, dp, lp, NlfemError
+from ._lib import lib, check, dp, lp, NlfemError, _fp
 from .config import parse
 from .quadrature import tensorgauss  # noqa: F401  (re-exported, cython/nlfem.pyx:108-126)
 from .mesh import mesh_dict
@@ -93,6 +93,39 @@ class Assembler:
                                   a.is_fullConnectedComponentSearch,
                                   _PRECISION[configuration.get("precision", "fp64")]))
         self._h = h
+        sur = configuration.get("surrogate")
+        if sur:
+            self.set_surrogate(**(sur if isinstance(sur, dict) else {}))
+
+    def set_surrogate(self, model="default", tol=None, smin=None, smax=None, enabled=True):
+        """Route cut pairs through the surrogate networks of `model` (a path written by
+        nlfem_amd.surrogate.save, or "default"); pairs outside the networks' domain or with an
+        ensemble spread above `tol` stay with the exact retriangulation."""
+        from . import surrogate
+        if not enabled:
+            check(lib().nlfem_plan_set_surrogate(self._h, 0, None, None, 0.0, 0.0, 1.0))
+            return self
+        params, norm, meta = surrogate.load(model)
+        tol = float(meta.get("tol", 1e-3) if tol is None else tol)
+        smin = float(meta.get("smin", 0.0) if smin is None else smin)
+        smax = float(meta.get("smax", 1.0) if smax is None else smax)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        check(lib().nlfem_plan_set_surrogate(self._h, p.shape[0], p.ctypes.data_as(_fp), nm.ctypes.data_as(_fp),
+                                             tol, smin, smax))
+        return self
+
+    def surrogate_samples(self, maxn, stride=1, salt=0, smin=0.0, smax=1.0):
+        """Training samples (features [n,12], targets [n,9]) from the exact FP64 cut-pair sums of
+        the assembled plan, and the number of qualifying items."""
+        F = np.zeros((max(maxn, 1), 12), dtype=np.float32)
+        T = np.zeros((max(maxn, 1), 9), dtype=np.float32)
+        cnt = C.c_longlong()
+        check(lib().nlfem_plan_surrogate_samples(self._h, int(maxn), int(stride), int(salt) & 0xFFFFFFFF,
+                                                 float(smin), float(smax), F.ctypes.data_as(_fp),
+                                                 T.ctypes.data_as(_fp), C.byref(cnt)))
+        m = min(cnt.value, maxn)
+        return F[:m], T[:m], cnt.value
 
     def assemble(self):
         check(lib().nlfem_plan_assemble(self._h))
@@ -125,7 +158,7 @@ class Assembler:
 
     STAT_KEYS = ("pairs", "probes", "close", "interior", "cut", "bary", "nnz_nodes", "nnz", "cap", "max_list", "subtri",
                  "ms_dual", "ms_bfs", "ms_transpose", "ms_pattern", "ms_positions", "ms_close", "ms_interior",
-                 "ms_cut", "ms_compact")
+                 "ms_cut", "ms_compact", "sur_pairs", "sur_rejected", "sur_items")
 
     def stats(self):
         out = np.zeros(len(self.STAT_KEYS))
