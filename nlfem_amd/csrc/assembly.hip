@@ -1562,7 +1562,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
 constexpr int SUR_ITEMS = 256;
 constexpr int SUR_MAXPAIRS = 36;
 
-template <int KER>
+template <int KER, int NNET>
 __global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
     constexpr int OD = 1, NB = 9;
     extern __shared__ int ldss[];
@@ -1651,8 +1651,9 @@ __global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
             for (int tile = w; tile * 16 < npi; tile += 4) {
                 const int sl = tile * 16 + j;
                 const bool valid = sl < npi;
-                float o0[4] = {0, 0, 0, 0}, o1[4] = {0, 0, 0, 0};
-                for (int net = 0; net < a.sur.nnet; net++) {
+                float mo[4] = {0, 0, 0, 0}, sp = 0;
+#pragma unroll 1
+                for (int net = 0; net < NNET; net++) {
                     const float* nm = a.sur.norm + net * mlp::NNORM;
                     mlp::f32x4 xin = {0, 0, 0, 0};
                     if (valid && q < mlp::NIN) {
@@ -1663,21 +1664,20 @@ __global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
                                                            a.sur.bias + net * mlp::BIAS_PAD, xin, lane);
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        const int o = q + i < mlp::NOUT ? q + i : 0;
-                        const float v = y[i] * nm[mlp::NORM_OSD + o] + nm[mlp::NORM_OMU + o];
-                        if (net == 0) o0[i] = v;
-                        else o1[i] = v;
+                        const int oi = q + i < mlp::NOUT ? q + i : 0;
+                        const float v = y[i] * nm[mlp::NORM_OSD + oi] + nm[mlp::NORM_OMU + oi];
+                        if (net == 0) {
+                            mo[i] = v;
+                        } else {  // two networks: mean and largest disagreement
+                            if (q + i < mlp::NOUT) sp = fmaxf(sp, fabsf(mo[i] - v));
+                            mo[i] = 0.5f * (mo[i] + v);
+                        }
                     }
                 }
                 if (valid) {
-                    float sp = 0;
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        if (q + i >= mlp::NOUT) continue;
-                        const float v = a.sur.nnet > 1 ? 0.5f * (o0[i] + o1[i]) : o0[i];
-                        outp[sl][q + i] = v;
-                        if (a.sur.nnet > 1) sp = fmaxf(sp, fabsf(o0[i] - o1[i]));
-                    }
+                    for (int i = 0; i < 4; i++)
+                        if (q + i < mlp::NOUT) outp[sl][q + i] = mo[i];
                     spread[sl][lane >> 4] = sp;
                 }
             }
@@ -1721,7 +1721,9 @@ __global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
                         const float* y = outp[islot[it]];
 #pragma unroll
                         for (int qq = 0; qq < 6; qq++) {
-                            const float r = fminf(fmaxf(y[qq], 0.0f), 1.0f);
+                            // ratios of moments of a subset of bT lie in (0, 1]; the floor keeps
+                            // a partially covered item from producing an exact zero entry
+                            const float r = fminf(fmaxf(y[qq], 1e-7f), 1.0f);
                             sLP[qq] = cd * a.sur.q2[qq] * (double)r;
                         }
                         // M_a := sum_b S_ab keeps the partition of unity of the pair exact
@@ -2474,8 +2476,13 @@ void plan_assemble(Plan& P, const Estimates& est) {
             ns.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
             set_surrogate_args(P, ns.sur);
             const size_t ls = (size_t)ns.HN * 16;
-            if (ls > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur<KER_CONSTANT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
-            k_cut_sur<KER_CONSTANT><<<c.nE, 256, ls, st>>>(ns);
+            if (P.sur.nnet == 2) {
+                if (ls > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur<KER_CONSTANT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+                k_cut_sur<KER_CONSTANT, 2><<<c.nE, 256, ls, st>>>(ns);
+            } else {
+                if (ls > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur<KER_CONSTANT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+                k_cut_sur<KER_CONSTANT, 1><<<c.nE, 256, ls, st>>>(ns);
+            }
             HIPCHECK(hipGetLastError());
             nc.sur = ns.sur;
             nc.sur.list_in = 1;

@@ -84,15 +84,22 @@ __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 __device__ inline f32x4 mfma(bf16x4, bf16x4, f32x4 c) { return c; }
 #endif
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32).
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 __device__ inline float dsilu(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
+}
+// FP32 -> BF16 with round to nearest even, two values per v_cvt_pk_bf16_f32.
+__device__ inline short bf16_bits(float f) {
+    const __bf16 h = (__bf16)f;
+    return __builtin_bit_cast(short, h);
 }
 __device__ inline bf16x4 pack4(f32x4 v) {
     bf16x4 r;
-    r[0] = (short)to_bf16(v[0]); r[1] = (short)to_bf16(v[1]);
-    r[2] = (short)to_bf16(v[2]); r[3] = (short)to_bf16(v[3]);
+    r[0] = bf16_bits(v[0]); r[1] = bf16_bits(v[1]);
+    r[2] = bf16_bits(v[2]); r[3] = bf16_bits(v[3]);
     return r;
 }
 
