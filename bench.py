@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -77,6 +79,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the mixed-precision and surrogate+mixed timings of the same workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -133,6 +137,39 @@ def main():
         "phases_ms": {k[3:]: round(v, 3) for k, v in ms_phase.items()},
         "roofline": roof,
     }
+    if not args.no_variants and args.precision == "fp64":
+        # the same workload in mixed precision and with the cut-pair surrogate (constant kernel),
+        # each compared with the FP64 matrix just assembled
+        import scipy.sparse.linalg as spla
+        A64 = asm.csr()
+        nrm = spla.norm(A64)
+        variants = {"mixed": {"precision": "mixed"}}
+        if kernel["function"] == "constant":
+            variants["surrogate+mixed"] = {"precision": "mixed", "surrogate": {"model": "default"}}
+        out["variants"] = {}
+        for name, extra in variants.items():
+            c2 = dict(conf)
+            c2.update(extra)
+            print(f"[bench] variant {name}", file=sys.stderr, flush=True)
+            a2 = nlfem.Assembler(mesh, kernel, c2)
+            for i in range(args.warmup):
+                a2.assemble()
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                a2.assemble()
+            ms2 = (time.perf_counter() - t0) * 1000.0 / args.steps
+            st2 = a2.stats()
+            B = a2.csr()
+            a2.close()
+            v = {"ms_per_step": ms2, "value": st2["pairs"] / (ms2 / 1000.0), "speedup_vs_fp64": ms_step / ms2,
+                 "rel_err_fro_vs_fp64": float(spla.norm(B - A64) / nrm),
+                 "pattern_equal": bool(np.array_equal(B.indptr, A64.indptr) and np.array_equal(B.indices, A64.indices)),
+                 "ms_cut": round(st2["ms_cut"], 3), "ms_interior": round(st2["ms_interior"], 3)}
+            if "surrogate" in extra:
+                v["cut_pairs_by_surrogate"] = int(st2["sur_pairs"])
+                v["cut_pairs_exact_fallback"] = int(st2["sur_rejected"])
+                v["network_items"] = int(st2["sur_items"])
+            out["variants"][name] = v
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(mesh, kernel, conf, args.cpu_budget)

@@ -19,10 +19,11 @@ every triangle is valid, so the mesh is a function of (n, jitter, seed) alone.
 import numpy as np
 
 
-def unit_square(n, delta, jitter=0.0, seed=0, outdim=1, ansatz="CG"):
-    h = 1.0 / n
+def unit_square(n, delta, jitter=0.0, seed=0, outdim=1, ansatz="CG", lo=0.0, hi=1.0):
+    """Mesh of [lo, hi]^2 (default the unit square) with Omega = (lo+delta, hi-delta)^2."""
+    h = (hi - lo) / n
     ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
-    lattice = np.stack([ii.ravel() * h, jj.ravel() * h], axis=1).astype(np.float64)
+    lattice = lo + np.stack([ii.ravel() * h, jj.ravel() * h], axis=1).astype(np.float64)
     verts = lattice.copy()
 
     sq_i, sq_j = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
@@ -55,7 +56,7 @@ def unit_square(n, delta, jitter=0.0, seed=0, outdim=1, ansatz="CG"):
         else:
             raise RuntimeError("jitter resampling did not converge")
 
-    inside = np.all((lattice > delta + 1e-9) & (lattice < 1.0 - delta - 1e-9), axis=1)
+    inside = np.all((lattice > lo + delta + 1e-9) & (lattice < hi - delta - 1e-9), axis=1)
     vlabels = np.where(inside, 1, -1).astype(np.int64)
     elabels = np.where((vlabels[elems] > 0).any(axis=1), 1, -1).astype(np.int64)
     return mesh_dict(elems, elabels, verts, vlabels, outdim=outdim, ansatz=ansatz, diam=np.sqrt(2.0) * h * (1 + 2 * jitter))

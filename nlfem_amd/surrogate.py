@@ -134,7 +134,7 @@ def load(path=None):
 
 # Domain of the network input size s = sqrt|det bT| / delta (training meshes cover about
 # 0.03 .. 0.3) and the default acceptance threshold on the ensemble spread.
-SMIN, SMAX, TOL = 0.02, 0.35, 5e-3
+SMIN, SMAX, TOL = 0.02, 0.2, 5e-3
 
 
 def training_meshes(count=20, seed=0):
@@ -204,7 +204,7 @@ def build_default(out, count=20, per_mesh=100_000, epochs=30, batch=1024, lr=3e-
             log(f"validation tol {tol:.0e}: accept 0")
     save(out, np.stack(P), np.stack(N), smin=SMIN, smax=SMAX, tol=TOL, kernel="constant", features="v1",
          train_items=len(X), train_meshes=count, epochs=epochs, batch=batch, lr=lr,
-         loss0=np.stack(H)[:, -1], val_err_mean=float(err.mean()))
+         final_loss=np.stack(H)[:, -1], val_err_mean=float(err.mean()))
     return out
 
 
