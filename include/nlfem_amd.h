@@ -100,6 +100,11 @@ int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
  * one of its items lies outside [smin, smax] (sqrt|det bT| / delta) or the two networks
  * disagree by more than tol on an output ratio.  Stats gain (after the phase times) the
  * pairs taken by the surrogate, the pairs handed back, and the network items evaluated. */
+/* Override the mesh-size estimates (longest edge, smallest and mean element area) from which
+ * the BFS and pattern table capacities are derived; for meshes made of several disconnected
+ * parts of different resolution, pass the values of the part with the most elements per
+ * horizon.  Capacities still grow on overflow. */
+int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double amean);
 int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm, float tol,
                              double smin, double smax);
 /* Training samples from the exact FP64 cut-pair sums of an assembled plan: feat[maxn][12],

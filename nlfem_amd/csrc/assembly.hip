@@ -258,7 +258,7 @@ struct BfsArgs {
     const int* adjp;  // dual graph padded to W4*4 entries per row (-1 = none), or null
     int W4;           // int4 chunks per padded row (0: use the CSR)
     QuadArgs q;
-    double sqdelta, sq_in;
+    double sqdelta, sq_in, sq_cut, sq_far;
     int caps, remote, level0, fullCC;
     int cap, H, F;
     uint32_t* nbr;
@@ -300,13 +300,19 @@ NLFEM_DEV void outer_points(const BfsArgs& a, const ElemRec& A, OuterPts& op) {
 }
 
 // Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
-// Returns the pair class or -1 when the pair does not interact.  Two conservative pre-tests
-// settle most candidates without a retriangulation; their margins (1e-7, 1e-6 relative) are
-// far above rounding, so they only ever agree with the exact tests below:
+// classify_quick returns the pair class, -1 when the pair does not interact, or CLS_HARD when
+// only retriangulations can tell (classify_full).  The conservative tests use margins (1e-7,
+// 1e-6, 1e-9 relative) far above rounding, so they only ever agree with the exact test:
 //  - every vertex of bT within delta(1-1e-7) - r of the outer points' centre: interior;
 //  - the bounding circle of bT farther than delta(1+1e-6) + r: every vertex is outside every
-//    ball and no edge reaches one, so the reference's polygon is empty for every k.
-NLFEM_DEV int classify(const BfsArgs& a, const OuterPts& op, const ElemRec& B, int method) {
+//    ball and no edge reaches one, so the reference's polygon is empty for every k;
+//  - every vertex within sq_in of every outer point: interior (the polygon is bT itself);
+//  - a vertex strictly inside some ball (squared distance <= sq_cut): it is a polygon point of
+//    that ball and each of its two edges leaves the ball once, so the polygon has >= 3
+//    points and the pair interacts.
+constexpr int CLS_HARD = -2;
+
+NLFEM_DEV int classify_quick(const BfsArgs& a, const OuterPts& op, const ElemRec& B, int method) {
     if (method == 2) return (int)CLS_CLOSE;
     if (method == M_BARYCENTER) {
         for (int k = 0; k < a.q.nPx; k++)
@@ -323,30 +329,63 @@ NLFEM_DEV int classify(const BfsArgs& a, const OuterPts& op, const ElemRec& B, i
         const double rb = sqrt(rb2) * (1 + 1e-12);
         if (sqrt(sqdist(b0, b1, op.c0, op.c1)) - rb > op.rout) return a.fullCC ? (int)CLS_CUT : -1;
     }
-    bool interior = true;
-    for (int k = 0; k < a.q.nPx && interior; k++)
-        for (int j = 0; j < 3; j++)
-            if (!(sqdist(B.E[2 * j], B.E[2 * j + 1], op.x[2 * k], op.x[2 * k + 1]) <= a.sq_in)) interior = false;
+    bool interior = true, inside = false;
+    for (int k = 0; k < a.q.nPx; k++)
+        for (int j = 0; j < 3; j++) {
+            const double d = sqdist(B.E[2 * j], B.E[2 * j + 1], op.x[2 * k], op.x[2 * k + 1]);
+            interior = interior && (d <= a.sq_in);
+            inside = inside || (d <= a.sq_cut);
+        }
     if (interior) return (int)CLS_INTERIOR;
+    if (inside) return (int)CLS_CUT;
+    return CLS_HARD;
+}
+
+// Squared distance from (x0, x1) to the segment P -> Q.
+NLFEM_DEV double seg_sqdist(double x0, double x1, double P0, double P1, double Q0, double Q1) {
+    const double D0 = Q0 - P0, D1 = Q1 - P1;
+    const double dd = D0 * D0 + D1 * D1;
+    double t = dd > 0 ? ((x0 - P0) * D0 + (x1 - P1) * D1) / dd : 0.0;
+    t = t < 0 ? 0 : (t > 1 ? 1 : t);
+    const double c0 = P0 + t * D0 - x0, c1 = P1 + t * D1 - x1;
+    return c0 * c0 + c1 * c1;
+}
+
+// The reference predicate (some outer point's polygon has >= 3 points) by retriangulation.
+// An outer point whose ball stays farther than delta (1 + 1e-9) from all three edges of bT
+// has no polygon point and is skipped.
+NLFEM_DEV int classify_full(const BfsArgs& a, const OuterPts& op, const ElemRec& B) {
     for (int k = 0; k < a.q.nPx; k++) {
+        const double x0 = op.x[2 * k], x1 = op.x[2 * k + 1];
+        const double dmin = fmin(seg_sqdist(x0, x1, B.E[0], B.E[1], B.E[2], B.E[3]),
+                                 fmin(seg_sqdist(x0, x1, B.E[2], B.E[3], B.E[4], B.E[5]),
+                                      seg_sqdist(x0, x1, B.E[4], B.E[5], B.E[0], B.E[1])));
+        if (dmin > a.sq_far) continue;
         NoEmit ne;
-        if (retriangulate<true>(op.x[2 * k], op.x[2 * k + 1], B.E, a.sqdelta, a.caps != 0, ne) >= 3)
-            return (int)CLS_CUT;
+        if (retriangulate<true>(x0, x1, B.E, a.sqdelta, a.caps != 0, ne) >= 3) return (int)CLS_CUT;
     }
     return a.fullCC ? (int)CLS_CUT : -1;
+}
+
+// Position of this lane among the lanes of the wavefront for which p holds; total = their number.
+NLFEM_DEV int wave_rank(bool p, int& total) {
+    const unsigned long long m = __ballot(p);
+    total = __popcll(m);
+    return __popcll(m & ((1ull << __lane_id()) - 1ull));
 }
 
 // One wavefront (one block of 64 lanes) per outer element.  Equivalent to the reference's
 // queue BFS (src/Cassemble.cpp:593-783): every element adjacent to an expanded element is
 // tested exactly once; interacting and label-0 elements are expanded; the first layer
 // {aT} u adj(aT) uses the close method.  The resulting set does not depend on visit order.
-__global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
+// All counters are wavefront-uniform registers, appended to with ballot ranks (no LDS atomics);
+// the candidates that only a retriangulation can settle are compacted in place and tested
+// densely after the cheap tests of the level.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_bfs(BfsArgs a) {
     extern __shared__ int lds[];
     int* hash = lds;
     int* test = hash + a.H;
     int* next = test + a.F;
-    __shared__ int nTest, nNext, nTot, nF, nB, ovf;
-    __shared__ unsigned long long st[6];
     __shared__ OuterPts op;
     const int lane = threadIdx.x;
     const int aT = blockIdx.x;
@@ -357,109 +396,158 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
         return;
     }
     for (int i = lane; i < a.H; i += 64) hash[i] = -1;
-    if (lane == 0) {
-        outer_points(a, A, op);
-        nTest = 0; nNext = 0; nTot = 0; nF = 0; nB = 0; ovf = 0;
-        for (int i = 0; i < 6; i++) st[i] = 0;
-    }
+    if (lane == 0) outer_points(a, A, op);
     __syncthreads();
+    int nTest = 0, nNext = 0, nTot = 0, nF = 0, nB = 0;
+    bool ovf = false;
+    unsigned long long cProbe = 0, cClose = 0, cInt = 0, cCut = 0, cBary = 0;
     // level 0 candidates: aT itself, then its dual-graph neighbours
     const long long x0 = a.xadj[aT];
     const int deg = (int)(a.xadj[aT + 1] - x0);
     if (1 + deg > a.F) {
-        if (lane == 0) ovf = 1;
+        ovf = true;
     } else {
         if (lane == 0) { lds_insert(hash, a.H, aT); test[0] = aT; }
         __syncthreads();
+        bool lo = false;
         for (int j = lane; j < deg; j += 64) {
-            int b = a.adj[x0 + j];
-            if (lds_insert(hash, a.H, b) < 0) ovf = 1;
+            const int b = a.adj[x0 + j];
+            if (lds_insert(hash, a.H, b) < 0) lo = true;
             test[1 + j] = b;
         }
-        if (lane == 0) nTest = 1 + deg;
+        ovf = __any(lo);
+        nTest = 1 + deg;
     }
     __syncthreads();
-    int level = 0;
-    while (!ovf) {
-        // test the candidates of this level
-        const int method = (level == 0) ? a.level0 : a.remote;
-        for (int j = lane; j < nTest; j += 64) {
-            const int b = test[j];
-            const ElemRec B = a.elem[b];
-            bool push = false;
-            if (B.label == 0) {
-                push = true;  // label-0 elements are traversed only (src/Cassemble.cpp:773-775)
-            } else {
-                int cls = classify(a, op, B, method);
-                if (cls >= 0) {
-                    push = true;
-                    int t = atomicAdd(&nTot, 1);
-                    if (t >= a.cap) {
-                        ovf = 1;
-                    } else {
-                        int slot;
-                        if (cls == (int)CLS_INTERIOR || cls == (int)CLS_CLOSE) slot = atomicAdd(&nF, 1);
-                        else slot = a.cap - 1 - atomicAdd(&nB, 1);
-                        uint32_t code = (cls == 3) ? CLS_CUT : (uint32_t)cls;
-                        a.nbr[(size_t)aT * a.cap + slot] = (uint32_t)b | (code << CLS_SHIFT) | (cls == 3 ? (1u << 31) : 0u);
-                    }
-                    atomicAdd(&st[cls == 3 ? 5 : (cls == (int)CLS_CLOSE ? 2 : (cls == (int)CLS_INTERIOR ? 3 : 4))], 1ull);
-                    if (cls == (int)CLS_CLOSE) {
-                        unsigned long long c = atomicAdd(&a.counters[CNT_CLOSE], 1ull);
-                        if ((long long)c < a.closeCap) a.closeList[c] = make_int2(aT, b);
-                    }
-                } else {
-                    atomicAdd(&st[1], 1ull);
-                }
-            }
-            if (push && b != aT) {
-                int q = atomicAdd(&nNext, 1);
-                if (q < a.F) next[q] = b;
-                else ovf = 1;
+
+    // record a tested candidate (cls >= 0: interacting) and queue it for expansion
+    auto record = [&](int b, int cls, bool push) {
+        const bool rec = cls >= 0;
+        const bool isF = cls == (int)CLS_INTERIOR || cls == (int)CLS_CLOSE;
+        int nr, nf, nb;
+        wave_rank(rec, nr);
+        const int rf = wave_rank(rec && isF, nf);
+        const int rb = wave_rank(rec && !isF, nb);
+        if (nTot + nr > a.cap) ovf = true;
+        if (rec && !ovf) {
+            const int slot = isF ? nF + rf : a.cap - 1 - (nB + rb);
+            const uint32_t code = (cls == 3) ? CLS_CUT : (uint32_t)cls;
+            a.nbr[(size_t)aT * a.cap + slot] = (uint32_t)b | (code << CLS_SHIFT) | (cls == 3 ? (1u << 31) : 0u);
+            if (cls == (int)CLS_CLOSE) {
+                const unsigned long long c = atomicAdd(&a.counters[CNT_CLOSE], 1ull);
+                if ((long long)c < a.closeCap) a.closeList[c] = make_int2(aT, b);
             }
         }
+        nTot += nr; nF += nf; nB += nb;
+        cClose += cls == (int)CLS_CLOSE; cInt += cls == (int)CLS_INTERIOR; cCut += cls == (int)CLS_CUT;
+        cBary += cls == 3;
+        const bool q = push && b != aT;
+        int nq;
+        const int rq = wave_rank(q, nq);
+        if (nNext + nq > a.F) ovf = true;
+        else if (q) next[nNext + rq] = b;
+        nNext += nq;
+    };
+
+    int level = 0;
+    while (!ovf) {
+        const int method = (level == 0) ? a.level0 : a.remote;
+        // cheap tests; undecided candidates are compacted to the front of test[] (every index
+        // written lies below the end of the round, whose entries have all been read)
+        int nHard = 0;
+        for (int r0 = 0; r0 < nTest; r0 += 64) {
+            const int j = r0 + lane;
+            const bool valid = j < nTest;
+            const int b = valid ? test[j] : -1;
+            int cls = -1;
+            bool push = false, hard = false;
+            if (valid) {
+                const ElemRec B = a.elem[b];
+                if (B.label == 0) {
+                    push = true;  // label-0 elements are traversed only (src/Cassemble.cpp:773-775)
+                } else {
+                    cls = classify_quick(a, op, B, method);
+                    hard = cls == CLS_HARD;
+                    if (hard) cls = -1;
+                    push = cls >= 0;
+                    cProbe += !hard && cls < 0;  // tested and not interacting
+                }
+            }
+            int nh;
+            const int rh = wave_rank(hard, nh);
+            if (hard) test[nHard + rh] = b;
+            nHard += nh;
+            record(b, cls, push);
+        }
+        __syncthreads();
+        for (int r0 = 0; r0 < nHard; r0 += 64) {
+            const int j = r0 + lane;
+            const bool valid = j < nHard;
+            const int b = valid ? test[j] : -1;
+            int cls = -1;
+            if (valid) {
+                cls = classify_full(a, op, a.elem[b]);
+                cProbe += cls < 0;
+            }
+            record(b, cls, cls >= 0);
+        }
+        ovf = __any(ovf);
         __syncthreads();
         if (nNext == 0 || ovf) break;
-        if (lane == 0) nTest = 0;
-        __syncthreads();
         // discover the next level: unvisited neighbours of the expanded elements
-        const int nn = nNext;
-        auto visit = [&](int e) {
-            int r = lds_insert(hash, a.H, e);
-            if (r < 0) ovf = 1;
-            else if (r == 1) {
-                int q = atomicAdd(&nTest, 1);
-                if (q < a.F) test[q] = e;
-                else ovf = 1;
+        nTest = 0;
+        bool lovf = false;
+        auto visit = [&](int e) {  // wavefront-uniform call; e < 0 for no candidate
+            bool isnew = false;
+            if (e >= 0) {
+                const int r = lds_insert(hash, a.H, e);
+                if (r < 0) lovf = true;
+                isnew = r == 1;
             }
+            int nt;
+            const int rt = wave_rank(isnew, nt);
+            if (isnew && nTest + rt < a.F) test[nTest + rt] = e;
+            nTest += nt;
         };
-        for (int j = lane; j < nn; j += 64) {
-            const int s = next[j];
+        for (int r0 = 0; r0 < nNext; r0 += 64) {
+            const int j = r0 + lane;
+            const int s = j < nNext ? next[j] : -1;
             if (a.W4 > 0) {
                 // padded row: all loads issued before the first insert
-                const int4* row = reinterpret_cast<const int4*>(a.adjp) + (size_t)s * a.W4;
+                const int4* row = reinterpret_cast<const int4*>(a.adjp) + (size_t)(s < 0 ? 0 : s) * a.W4;
                 int4 v[8];
 #pragma unroll
                 for (int c = 0; c < 8; c++)
-                    if (c < a.W4) v[c] = row[c];
+                    if (c < a.W4) v[c] = s >= 0 ? row[c] : make_int4(-1, -1, -1, -1);
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
                     if (c >= a.W4) break;
-                    if (v[c].x >= 0) visit(v[c].x);
-                    if (v[c].y >= 0) visit(v[c].y);
-                    if (v[c].z >= 0) visit(v[c].z);
-                    if (v[c].w >= 0) visit(v[c].w);
+                    visit(v[c].x);
+                    visit(v[c].y);
+                    visit(v[c].z);
+                    visit(v[c].w);
                 }
             } else {
-                for (long long p = a.xadj[s]; p < a.xadj[s + 1]; p++) visit(a.adj[p]);
+                const long long p0 = s >= 0 ? a.xadj[s] : 0;
+                const int len = s >= 0 ? (int)(a.xadj[s + 1] - p0) : 0;
+                int mlen = len;
+                for (int m = 32; m > 0; m >>= 1) mlen = max(mlen, __shfl_xor(mlen, m, 64));
+                for (int p = 0; p < mlen; p++) visit(p < len ? a.adj[p0 + p] : -1);
             }
         }
-        __syncthreads();
-        if (lane == 0) nNext = 0;
+        ovf = __any(lovf) || nTest > a.F;
+        nNext = 0;
         __syncthreads();
         level++;
     }
-    __syncthreads();
+    // wave reductions of the statistics
+    for (int m = 32; m > 0; m >>= 1) {
+        cProbe += __shfl_xor(cProbe, m, 64);
+        cClose += __shfl_xor(cClose, m, 64);
+        cInt += __shfl_xor(cInt, m, 64);
+        cCut += __shfl_xor(cCut, m, 64);
+        cBary += __shfl_xor(cBary, m, 64);
+    }
     if (lane == 0) {
         if (ovf) {
             atomicOr(&a.counters[CNT_OVERFLOW], 1ull);
@@ -470,11 +558,11 @@ __global__ __launch_bounds__(64) void k_bfs(BfsArgs a) {
             a.cntB[aT] = nB;
             atomicMax(&a.counters[CNT_MAXLIST], (unsigned long long)(nF + nB));
         }
-        atomicAdd(&a.counters[CNT_PAIRS], st[2] + st[3] + st[4] + st[5]);
-        atomicAdd(&a.counters[CNT_PROBES], st[1]);
-        atomicAdd(&a.counters[CNT_INTERIOR], st[3]);
-        atomicAdd(&a.counters[CNT_CUT], st[4]);
-        atomicAdd(&a.counters[CNT_BARY], st[5]);
+        atomicAdd(&a.counters[CNT_PAIRS], cClose + cInt + cCut + cBary);
+        atomicAdd(&a.counters[CNT_PROBES], cProbe);
+        atomicAdd(&a.counters[CNT_INTERIOR], cInt);
+        atomicAdd(&a.counters[CNT_CUT], cCut);
+        atomicAdd(&a.counters[CNT_BARY], cBary);
     }
 }
 
@@ -2218,6 +2306,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
     ba.adjp = P.d_adjp; ba.W4 = W4;
     ba.sqdelta = c.sqdelta;
     ba.sq_in = c.sqdelta * (1.0 - 1e-9);
+    ba.sq_cut = c.sqdelta * (1.0 - 1e-9);
+    ba.sq_far = c.sqdelta * (1.0 + 1e-9);
     ba.caps = c.caps;
     ba.remote = c.method_remote;
     ba.level0 = c.singular ? (c.method_close >= 2 ? 2 : c.method_close) : c.method_remote;

@@ -298,6 +298,16 @@ extern "C" int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n) {
     return NLFEM_OK;
 }
 
+extern "C" int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double amean) {
+    if (!plan) return fail(NLFEM_EINVAL, "null plan");
+    if (!(hmax > 0) || !(amin > 0) || !(amean >= amin)) return fail(NLFEM_EINVAL, "invalid size estimates");
+    plan->est.hmax = hmax;
+    plan->est.amin = amin;
+    plan->est.amean = amean;
+    plan->P.bfs_visit = 0;  // re-derive the BFS capacities from the new estimates
+    return NLFEM_OK;
+}
+
 extern "C" int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm,
                                         float tol, double smin, double smax) {
     if (!plan) return fail(NLFEM_EINVAL, "null plan");
