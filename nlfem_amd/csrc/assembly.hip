@@ -756,6 +756,7 @@ struct SurArgs {
 
 struct NumArgs {
     int nE, cap, is_DG, caps, HN, od;
+    double sq_in, sq_far;  // sqdelta (1 - 1e-9), sqdelta (1 + 1e-9)
     const ElemRec* elem;
     const int* tri;
     const int* vlabel;
@@ -1451,6 +1452,38 @@ struct PointCollector {
     }
 };
 
+// Outer accumulation of a chunk of cut pairs from the per-item inner sums iacc (10*OO per
+// item: innerLocal, innerNonloc, the six psi_a psi_b products), as outer_point and add_lp
+// do it per pair (src/integration.cpp:508-537), with one thread per (pair, entry):
+// termNonloc and termLocalPrime into pacc.  termLocal only goes to the outer element's own
+// rows, so tls[k*OO + o] collects innerLocal over all pairs of the block per outer point.
+template <int OD>
+NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* iacc, double* pacc, double* tls) {
+    constexpr int OO = OD * OD, NB = 9 * OO;
+    const int nPx = a.q.nPx;
+    const int t = threadIdx.x;
+    if (t < np) {
+        double tN[NB], tLP[NB];
+#pragma unroll
+        for (int o = 0; o < NB; o++) { tN[o] = 0; tLP[o] = 0; }
+        for (int k = 0; k < nPx; k++) {
+            const double* ia = iacc + ((size_t)t * nPx + k) * 10 * OO;
+            double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
+            outer_point<OD, false>(psx, absA * a.q.dx[k], ia, ia + OO, nullptr, tN);
+            add_lp<OD>(ia + 4 * OO, a.q.dx[k] * absA, tLP);
+        }
+        double* pa = pacc + t * 2 * NB;
+#pragma unroll
+        for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
+    } else if (t < np + nPx * OO) {
+        // innerLocal summed over the chunk's pairs per (outer point, component)
+        const int k = (t - np) / OO, o = (t - np) % OO;
+        double v = 0;
+        for (int p = 0; p < np; p++) v += iacc[((size_t)p * nPx + k) * 10 * OO + o];
+        tls[k * OO + o] += v;
+    }
+}
+
 // MODE 0 assembles; MODE 1 writes training samples of the surrogate (partial items, exact
 // FP64 inner sums as targets) and assembles nothing.
 template <int KER, int OD, typename T, int MODE = 0>
@@ -1466,7 +1499,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     __shared__ int tpre[CUT_ITEMS + 1];
     __shared__ int pb_[CUT_MAXPAIRS];                               // inner element of each chunk pair
     __shared__ PivotLU lus[CUT_MAXPAIRS];
-    __shared__ double red[4][NB];
+    __shared__ double tls[MAXQ * OO];
     __shared__ unsigned subtri;
     __shared__ int wsum[2];
     const int aT = blockIdx.x;
@@ -1485,6 +1518,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
     }
     for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    for (int i = tid; i < MAXQ * OO; i += 256) tls[i] = 0.0;
     if (tid == 0) subtri = 0;
     __syncthreads();
     for (int j = 0; j < 3; j++) {
@@ -1496,9 +1530,6 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             else atomicOr(&a.counters[CNT_ERROR], 16ull);
         }
     }
-    double tL[NB];
-#pragma unroll
-    for (int i = 0; i < NB; i++) tL[i] = 0;
     const double absA = A.absDet;
     unsigned mytri = 0;
     for (int c0 = 0; c0 < nB; c0 += CP) {
@@ -1513,17 +1544,25 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             lus[tid] = pivot_lu(a.elem[b].E);
         }
         __syncthreads();
-        // A: polygons
+        // A: polygons; an outer point whose ball stays farther than delta (1 + 1e-9) from all
+        // three edges of bT has an empty polygon and skips the retriangulation.
         if (tid < nit) {
             const int p = tid / nPx, k = tid % nPx;
             const ElemRec B = a.elem[pb_[p]];
             double x[2];
             toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
-            PointCollector pc{pts + tid * CUT_MAXP, 0};
-            retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
-            if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
-            ntri[tid] = min(pc.nt, CUT_MAXP - 2);
-            mytri += ntri[tid];
+            int nt = 0;
+            const double dmin = fmin(seg_sqdist(x[0], x[1], B.E[0], B.E[1], B.E[2], B.E[3]),
+                                     fmin(seg_sqdist(x[0], x[1], B.E[2], B.E[3], B.E[4], B.E[5]),
+                                          seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
+            if (!(dmin > a.sq_far)) {
+                PointCollector pc{pts + tid * CUT_MAXP, 0};
+                retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
+                if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
+                nt = min(pc.nt, CUT_MAXP - 2);
+            }
+            ntri[tid] = nt;
+            mytri += nt;
         } else if (tid < CUT_ITEMS) {
             ntri[tid] = 0;
         }
@@ -1610,21 +1649,9 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             }
             continue;
         }
-        // C: outer accumulation (src/integration.cpp:508-537), one thread per pair
-        if (tid < np) {
-            double tN[NB], tLP[NB];
-#pragma unroll
-            for (int o = 0; o < NB; o++) { tN[o] = 0; tLP[o] = 0; }
-            for (int k = 0; k < nPx; k++) {
-                const double* ia = iacc + (tid * nPx + k) * 10 * OO;
-                double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
-                outer_point<OD>(psx, absA * a.q.dx[k], ia, ia + OO, tL, tN);
-                add_lp<OD>(ia + 4 * OO, a.q.dx[k] * absA, tLP);
-            }
-            double* pa = pacc + tid * 2 * NB;
-#pragma unroll
-            for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
-        }
+        // C: outer accumulation (src/integration.cpp:508-537), one thread per (pair, entry) of
+        // termNonloc, termLocalPrime and termLocal; termLocal is summed over the block in LDS
+        cut_outer<OD>(a, np, absA, iacc, pacc, tls);
         __syncthreads();
         // D: scatter (src/Cassemble.cpp:721-770)
         cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
@@ -1633,8 +1660,17 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     atomicAdd(&subtri, mytri);
     __syncthreads();
     if (tid == 0) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)subtri);
-    // termLocal: block reduction, then into (aAdx[i], aAdx[j])
-    cut_flush_local<OD>(a, aT, tL, labA, red);
+    // termLocal[N ra + cb] = sum_k |aT| dx_k psi_a(x_k) psi_b(x_k) innerLocal_k, into (aAdx[i], aAdx[j])
+    for (int t = tid; t < NB; t += 256) {
+        const int ra = t / N, cb = t % N;
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (!labA[i]) continue;
+        double v = 0;
+        for (int k = 0; k < nPx; k++)
+            v += absA * a.q.dx[k] * a.q.psix[3 * k + i] * a.q.psix[3 * k + j] * tls[k * OO + OD * ao + bo];
+        const long long p = a.posTT[9 * aT + 3 * i + j];
+        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
 }
 
 // Cut pairs through the surrogate (constant kernel, scalar), one 256-thread block per outer
@@ -2212,6 +2248,8 @@ static NumArgs base_numargs(const Plan& P) {
     na.elem = P.d_elem; na.tri = P.d_tri; na.vlabel = P.d_vlabel; na.nbr = P.d_nbr; na.cntF = P.d_cntF;
     na.cntB = P.d_cntB; na.rowptr = P.d_rowptr; na.col = P.d_col; na.tpos = P.d_tpos; na.posTT = P.d_posTT;
     na.val = P.d_val; na.q = c.q; na.kp = kernel_params(c); na.counters = P.d_counters;
+    na.sq_in = c.sqdelta * (1.0 - 1e-9);
+    na.sq_far = c.sqdelta * (1.0 + 1e-9);
     na.sur = SurArgs{};
     na.sur.stride = 1;
     return na;
