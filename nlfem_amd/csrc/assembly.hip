@@ -757,6 +757,11 @@ struct SurArgs {
 struct NumArgs {
     int nE, cap, is_DG, caps, HN, od;
     double sq_in, sq_far;  // sqdelta (1 - 1e-9), sqdelta (1 + 1e-9)
+    // sums of the outer and inner rules (constant kernel: every block of an interior pair
+    // factorises into c |aT| |bT| times products of these):
+    // p1_a = sum_k dx_k psi_a(P_k), p2_ab = sum_k dx_k psi_a psi_b (P_k) (a <= b), pdx = sum_k dx_k,
+    // q0 = sum_i dy_i, q1_b = sum_i dy_i psi_b(P_i), q2_ab = sum_i dy_i psi_a psi_b (P_i)
+    double p1[3], p2[6], pdx, q0, q1[3], q2[6];
     const ElemRec* elem;
     const int* tri;
     const int* vlabel;
@@ -1233,9 +1238,46 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     double tL[NB], tLP[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) { tL[i] = 0; tLP[i] = 0; }
+    // constant scalar kernel: the blocks of an interior pair are c |aT| |bT| times rule sums
+    // (termNonloc_ab = p1_a q1_b, termLocal_ab = q0 p2_ab, termLocalPrime_ab = pdx q2_ab), so
+    // per pair only |bT| is needed; termLocal and termLocalPrime take the sum of |bT|
+    constexpr bool FACT = (KER == KER_CONSTANT && OD == 1);
+    double sumB = 0, sumU = 0;
     // T as outer element: front segment, interior class
     const int nF = a.cntF[T0];
-    for (int p = tid; p < nF; p += 256) {
+    if constexpr (FACT) {
+        for (int p = tid; p < nF; p += 256) {
+            const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
+            if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const double w = a.kp.c * A.absDet * a.elem[U].absDet;
+            sumB += a.elem[U].absDet;
+            for (int j = 0; j < 3; j++) {
+                const int s = lds_find(hk, a.HN, node_of(a.tri, U, j, a.is_DG));
+                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+                const double wq = w * a.q1[j];
+                for (int i = 0; i < 3; i++)
+                    if (labA[i]) atomicAdd(&acc[s * 3 + i], -(wq * a.p1[i]));
+            }
+        }
+        const long long tb = a.tptr[T0], te = a.tptr[T0 + 1];
+        for (long long q = tb + tid; q < te; q += 256) {
+            const uint32_t e = (uint32_t)a.tadj[q];
+            if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const double absU = a.elem[U].absDet;
+            const double w = a.kp.c * absU * A.absDet;
+            sumU += absU;
+            for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i): outer U, inner T
+                const int s = lds_find(hk, a.HN, node_of(a.tri, U, i, a.is_DG));
+                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+                const double wp = w * a.p1[i];
+                for (int j = 0; j < 3; j++)
+                    if (labA[j]) atomicAdd(&acc[s * 3 + j], -(wp * a.q1[j]));
+            }
+        }
+    }
+    for (int p = tid; p < (FACT ? 0 : nF); p += 256) {
         const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
         if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
         const int U = (int)(e & IDX_MASK);
@@ -1256,7 +1298,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         }
     }
     // T as inner element: pairs (U, T) with T in the interior front segment of U
-    const long long tb = a.tptr[T0], te = a.tptr[T0 + 1];
+    const long long tb = a.tptr[T0], te = FACT ? a.tptr[T0] : a.tptr[T0 + 1];
     for (long long q = tb + tid; q < te; q += 256) {
         const uint32_t e = (uint32_t)a.tadj[q];
         if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
@@ -1278,6 +1320,13 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                         atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -pb.tN[N * (i * OD + ao) + j * OD + bo]);
             }
         }
+    }
+    if constexpr (FACT) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                tL[3 * i + j] = a.kp.c * A.absDet * sumB * a.q0 * a.p2[sym6(i, j)];
+                tLP[3 * i + j] = a.kp.c * sumU * A.absDet * a.pdx * a.q2[sym6(i, j)];
+            }
     }
     // block reduction of termLocal and termLocalPrime
     const int lane = tid & 63, w = tid >> 6;
@@ -2250,6 +2299,26 @@ static NumArgs base_numargs(const Plan& P) {
     na.val = P.d_val; na.q = c.q; na.kp = kernel_params(c); na.counters = P.d_counters;
     na.sq_in = c.sqdelta * (1.0 - 1e-9);
     na.sq_far = c.sqdelta * (1.0 + 1e-9);
+    {
+        const QuadArgs& q = c.q;
+        na.pdx = 0; na.q0 = 0;
+        for (int i = 0; i < 3; i++) { na.p1[i] = 0; na.q1[i] = 0; }
+        for (int i = 0; i < 6; i++) { na.p2[i] = 0; na.q2[i] = 0; }
+        for (int k = 0; k < q.nPx; k++) {
+            const double* ps = &q.psix[3 * k];
+            const double pp[6] = {ps[0] * ps[0], ps[0] * ps[1], ps[0] * ps[2], ps[1] * ps[1], ps[1] * ps[2], ps[2] * ps[2]};
+            na.pdx += q.dx[k];
+            for (int b = 0; b < 3; b++) na.p1[b] += q.dx[k] * ps[b];
+            for (int m = 0; m < 6; m++) na.p2[m] += q.dx[k] * pp[m];
+        }
+        for (int i = 0; i < q.nPy; i++) {
+            const double* ps = &q.psiy[3 * i];
+            const double pp[6] = {ps[0] * ps[0], ps[0] * ps[1], ps[0] * ps[2], ps[1] * ps[1], ps[1] * ps[2], ps[2] * ps[2]};
+            na.q0 += q.dy[i];
+            for (int b = 0; b < 3; b++) na.q1[b] += q.dy[i] * ps[b];
+            for (int m = 0; m < 6; m++) na.q2[m] += q.dy[i] * pp[m];
+        }
+    }
     na.sur = SurArgs{};
     na.sur.stride = 1;
     return na;
