@@ -762,6 +762,7 @@ struct NumArgs {
     // p1_a = sum_k dx_k psi_a(P_k), p2_ab = sum_k dx_k psi_a psi_b (P_k) (a <= b), pdx = sum_k dx_k,
     // q0 = sum_i dy_i, q1_b = sum_i dy_i psi_b(P_i), q2_ab = sum_i dy_i psi_a psi_b (P_i)
     double p1[3], p2[6], pdx, q0, q1[3], q2[6];
+    double qm[5];  // inner-rule moments: sum_i dy_i (P0, P1, P0 P0, P0 P1, P1 P1)
     const ElemRec* elem;
     const int* tri;
     const int* vlabel;
@@ -1506,6 +1507,55 @@ struct PointCollector {
 // do it per pair (src/integration.cpp:508-537), with one thread per (pair, entry):
 // termNonloc and termLocalPrime into pacc.  termLocal only goes to the outer element's own
 // rows, so tls[k*OO + o] collects innerLocal over all pairs of the block per outer point.
+// Inner sums of one outer point for the constant kernel, accumulated from the fan triangles
+// (R0, R_j, R_j+1) as the retriangulation emits them.  With the basis of bT affine on each fan
+// triangle, the inner rule's sums of 1, psi_b and psi_a psi_b over the triangle's
+// pulled-back points equal the rule's moments (q0, qm) applied to the basis values at the
+// triangle's corners, so each corner is pulled back once (the reference's pivoted toRef)
+// instead of each inner point; the results agree with the point-by-point sums to rounding.
+struct ConstantItem {
+    const NumArgs& a;
+    const PivotLU& lu;
+    double inL = 0, inN[3] = {0, 0, 0}, sLP[6] = {0, 0, 0, 0, 0, 0};
+    double g0[3], gp[3];
+    int nt = 0;
+    NLFEM_DEV ConstantItem(const NumArgs& a_, const PivotLU& lu_) : a(a_), lu(lu_) {}
+    NLFEM_DEV void basis(double y0, double y1, double* g) const {
+        double p0, p1;
+        lu.solve(y0, y1, p0, p1);
+        g[0] = 1 - p0 - p1;
+        g[1] = p0;
+        g[2] = p1;
+    }
+    NLFEM_DEV void operator()(double r0, double r1, double s0, double s1, double t0, double t1) {
+#pragma clang fp contract(fast)
+        if (nt == 0) {
+            basis(r0, r1, g0);
+            basis(s0, s1, gp);
+        }
+        nt++;
+        double gn[3];
+        basis(t0, t1, gn);
+        const double rT[6] = {r0, r1, s0, s1, t0, t1};
+        const double w = a.kp.c * absDet(rT);
+        const double q0 = a.q0, mx = a.qm[0], my = a.qm[1], mxx = a.qm[2], mxy = a.qm[3], myy = a.qm[4];
+        double d1[3], d2[3];
+        for (int b = 0; b < 3; b++) {
+            d1[b] = gp[b] - g0[b];
+            d2[b] = gn[b] - g0[b];
+            gp[b] = gn[b];
+        }
+        inL += w * q0;
+        for (int b = 0; b < 3; b++) inN[b] += w * (g0[b] * q0 + d1[b] * mx + d2[b] * my);
+        int m = 0;
+        for (int u = 0; u < 3; u++)
+            for (int v = u; v < 3; v++, m++)
+                sLP[m] += w * (g0[u] * g0[v] * q0 + (g0[u] * d1[v] + d1[u] * g0[v]) * mx +
+                               (g0[u] * d2[v] + d2[u] * g0[v]) * my + d1[u] * d1[v] * mxx +
+                               (d1[u] * d2[v] + d2[u] * d1[v]) * mxy + d2[u] * d2[v] * myy);
+    }
+};
+
 template <int OD>
 NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* iacc, double* pacc, double* tls) {
     constexpr int OO = OD * OD, NB = 9 * OO;
@@ -1605,13 +1655,23 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
                                      fmin(seg_sqdist(x[0], x[1], B.E[2], B.E[3], B.E[4], B.E[5]),
                                           seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
             if (!(dmin > a.sq_far)) {
-                PointCollector pc{pts + tid * CUT_MAXP, 0};
-                retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
-                if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
-                nt = min(pc.nt, CUT_MAXP - 2);
+                if constexpr (KER == KER_CONSTANT && OD == 1) {  // integrated here: no fan triangles for phase B
+                    ConstantItem ci(a, lus[p]);
+                    retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, ci);
+                    mytri += ci.nt;
+                    double* ia = iacc + tid * 10;
+                    ia[0] = ci.inL;
+                    for (int q = 0; q < 3; q++) ia[1 + q] = ci.inN[q];
+                    for (int q = 0; q < 6; q++) ia[4 + q] = ci.sLP[q];
+                } else {
+                    PointCollector pc{pts + tid * CUT_MAXP, 0};
+                    retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
+                    if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
+                    nt = min(pc.nt, CUT_MAXP - 2);
+                    mytri += nt;
+                }
             }
             ntri[tid] = nt;
-            mytri += nt;
         } else if (tid < CUT_ITEMS) {
             ntri[tid] = 0;
         }
@@ -1719,6 +1779,152 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             v += absA * a.q.dx[k] * a.q.psix[3 * k + i] * a.q.psix[3 * k + j] * tls[k * OO + OD * ao + bo];
         const long long p = a.posTT[9 * aT + 3 * i + j];
         atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
+}
+
+// Cut pairs, scalar kernels with at most 8 outer points: one 256-thread block per outer
+// element; 32 groups of 8 lanes, one group per cut pair and one lane per outer point, so a
+// chunk of 32 pairs needs no block barrier.  Each lane retriangulates B(x_k, delta) n bT and
+// integrates the fan triangles as they are emitted (constant kernel: from the corners and
+// the rule's moments, cut_item_constant; otherwise the inner rule point by point, as
+// pair_cut), forms its outer point's share of termNonloc and termLocalPrime
+// (src/integration.cpp:508-537), and the group sums them with three xor shuffles.  The 9
+// (i, j) entries of the pair are scattered by the group's lanes (src/Cassemble.cpp:721-770).
+// termLocal only reaches the outer element's rows: each lane keeps innerLocal of its outer
+// point summed over all its pairs, and the block forms termLocal once at the end.
+template <int KER, typename T>
+__global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
+    extern __shared__ int ldsc8[];
+    int* hk = ldsc8;      // [HN]
+    int* hv = hk + a.HN;  // [HN][3]
+    __shared__ double sred[4][8];
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int g = tid >> 3, k = tid & 7;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nB = a.sur.list_in ? a.sur.rcnt[aT] : a.cntB[aT];
+    if (nB == 0) return;
+    const int nPx = a.q.nPx;
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256) {
+            int sl = lds_slot(hk, a.HN, a.col[r0 + i]);
+            if (sl >= 0) hv[3 * sl + j] = i;
+            else atomicOr(&a.counters[CNT_ERROR], 16ull);
+        }
+    }
+    __syncthreads();
+    const bool kin = k < nPx;
+    double x[2] = {0, 0};
+    if (kin) toPhys(A.E, a.q.Px[2 * (kin ? k : 0)], a.q.Px[2 * (kin ? k : 0) + 1], x);
+    const double ak = kin ? A.absDet * a.q.dx[k] : 0.0;   // |aT| dx_k
+    const double wk = kin ? a.q.dx[k] * A.absDet : 0.0;   // dx_k |aT| (add_lp's order)
+    double psx[3] = {0, 0, 0};
+    if (kin) for (int i = 0; i < 3; i++) psx[i] = a.q.psix[3 * k + i];
+    double sumL = 0;  // innerLocal of this lane's outer point over its pairs
+    unsigned mytri = 0;
+    for (int c0 = 0; c0 < nB; c0 += 32) {
+        const int p = c0 + g;
+        if (p >= nB) break;  // whole groups leave together
+        const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + p]
+                                    : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - p] & IDX_MASK);
+        const ElemRec B = a.elem[b];
+        double inL = 0, inN[3] = {0, 0, 0}, sLP[6] = {0, 0, 0, 0, 0, 0};
+        if (kin) {
+            const double dmin = fmin(seg_sqdist(x[0], x[1], B.E[0], B.E[1], B.E[2], B.E[3]),
+                                     fmin(seg_sqdist(x[0], x[1], B.E[2], B.E[3], B.E[4], B.E[5]),
+                                          seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
+            if (!(dmin > a.sq_far)) {
+                const PivotLU lu = pivot_lu(B.E);
+                if constexpr (KER == KER_CONSTANT) {
+                    ConstantItem ci(a, lu);
+                    retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, ci);
+                    mytri += ci.nt;
+                    inL = ci.inL;
+                    for (int q = 0; q < 3; q++) inN[q] = ci.inN[q];
+                    for (int q = 0; q < 6; q++) sLP[q] = ci.sLP[q];
+                } else {
+                    auto emit = [&](double r0, double r1, double s0, double s1, double t0, double t1) {
+                        const double rT[6] = {r0, r1, s0, s1, t0, t1};
+                        const double rTdet = absDet(rT);
+                        mytri++;
+                        for (int i = 0; i < a.q.nPy; i++) {
+                            double y[2];
+                            toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                            T K[1];
+                            kernel_eval<KER, 1, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+                            double p0, p1;
+                            lu.solve(y[0], y[1], p0, p1);
+                            T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
+                            inner_point<1, T>(K, (T)(a.q.dy[i] * rTdet), psi, &inL, inN, sLP);
+                        }
+                    };
+                    retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, emit);
+                }
+            }
+        }
+        sumL += inL;
+        // this outer point's share of termNonloc (9) and termLocalPrime (9), summed over the group
+        double tN[9], tLP[9];
+        {
+#pragma clang fp contract(fast)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    tN[3 * i + j] = ak * psx[i] * inN[j];
+                    tLP[3 * i + j] = wk * sLP[sym6(i, j)];
+                }
+        }
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                tN[e] += shfl_xor_d(tN[e], m);
+                tLP[e] += shfl_xor_d(tLP[e], m);
+            }
+        // scatter: lane k takes entry (i, j) = k (and lane 0 also entry 8)
+        for (int e = k; e < 9; e += 8) {
+            const int i = e / 3, j = e % 3;
+            const int rBj = node_of(a.tri, b, j, a.is_DG);
+            const bool labBj = row_labelled(a.vlabel, rBj, a.is_DG);
+            const int sl = lds_find(hk, a.HN, rBj);
+            if (sl < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
+            const long long pos = a.rowptr[rA[i]] + hv[3 * sl + i];  // entry (aAdx[i], bAdx[j])
+            if (labA[i]) atomicAdd(&a.val[pos], -tN[e]);
+            if (labBj) {
+                atomicAdd(&a.val[a.tpos[pos]], -tN[e]);
+                atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], tLP[e]);  // (bAdx[j], bAdx[i])
+            }
+        }
+    }
+    // termLocal[i][j] = sum_k |aT| dx_k psi_i(x_k) psi_j(x_k) innerLocal_k: sum over the lanes of
+    // one outer point (k = tid & 7), then over the waves
+    for (int m = 8; m < 64; m <<= 1) sumL += shfl_xor_d(sumL, m);
+    if (lane < 8) sred[w][lane] = sumL;
+    unsigned tri = mytri;
+    for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
+    if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
+    __syncthreads();
+    if (tid < 9) {
+        const int i = tid / 3, j = tid % 3;
+        if (labA[i]) {
+            double v = 0;
+            for (int kk = 0; kk < nPx; kk++) {
+                const double S = sred[0][kk] + sred[1][kk] + sred[2][kk] + sred[3][kk];
+                v += A.absDet * a.q.dx[kk] * a.q.psix[3 * kk + i] * a.q.psix[3 * kk + j] * S;
+            }
+            atomicAdd(&a.val[a.posTT[9 * aT + 3 * i + j]], v);
+        }
     }
 }
 
@@ -2311,12 +2517,19 @@ static NumArgs base_numargs(const Plan& P) {
             for (int b = 0; b < 3; b++) na.p1[b] += q.dx[k] * ps[b];
             for (int m = 0; m < 6; m++) na.p2[m] += q.dx[k] * pp[m];
         }
+        for (int m = 0; m < 5; m++) na.qm[m] = 0;
         for (int i = 0; i < q.nPy; i++) {
             const double* ps = &q.psiy[3 * i];
             const double pp[6] = {ps[0] * ps[0], ps[0] * ps[1], ps[0] * ps[2], ps[1] * ps[1], ps[1] * ps[2], ps[2] * ps[2]};
             na.q0 += q.dy[i];
             for (int b = 0; b < 3; b++) na.q1[b] += q.dy[i] * ps[b];
             for (int m = 0; m < 6; m++) na.q2[m] += q.dy[i] * pp[m];
+            const double P0 = q.Py[2 * i], P1 = q.Py[2 * i + 1];
+            na.qm[0] += q.dy[i] * P0;
+            na.qm[1] += q.dy[i] * P1;
+            na.qm[2] += q.dy[i] * P0 * P0;
+            na.qm[3] += q.dy[i] * P0 * P1;
+            na.qm[4] += q.dy[i] * P1 * P1;
         }
     }
     na.sur = SurArgs{};
@@ -2698,14 +2911,35 @@ void plan_assemble(Plan& P, const Estimates& est) {
             k_cut<KER, OD, double><<<c.nE, 256, lb, st>>>(nc);                                                       \
         }                                                                                                            \
     } while (0)
-        switch (c.kernel) {
-            case KER_CONSTANT: LAUNCH_CUT(KER_CONSTANT, 1); break;
-            case KER_PARABOLA: LAUNCH_CUT(KER_PARABOLA, 1); break;
-            case KER_FRACTIONAL: LAUNCH_CUT(KER_FRACTIONAL, 1); break;
-            case KER_MICRO: LAUNCH_CUT(KER_MICRO, 1); break;
-            default: LAUNCH_CUT(KER_MICRO_FIELD, 2); break;
+#define LAUNCH_CUT8(KER)                                                                                  \
+    do {                                                                                                  \
+        const size_t l8 = (size_t)nc.HN * 16;                                                             \
+        if (c.precision == 1) {                                                                           \
+            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
+            k_cut8<KER, float><<<c.nE, 256, l8, st>>>(nc);                                                \
+        } else {                                                                                          \
+            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
+            k_cut8<KER, double><<<c.nE, 256, l8, st>>>(nc);                                               \
+        }                                                                                                 \
+    } while (0)
+        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG) {
+            switch (c.kernel) {
+                case KER_CONSTANT: LAUNCH_CUT8(KER_CONSTANT); break;
+                case KER_PARABOLA: LAUNCH_CUT8(KER_PARABOLA); break;
+                case KER_FRACTIONAL: LAUNCH_CUT8(KER_FRACTIONAL); break;
+                default: LAUNCH_CUT8(KER_MICRO); break;
+            }
+        } else {
+            switch (c.kernel) {
+                case KER_CONSTANT: LAUNCH_CUT(KER_CONSTANT, 1); break;
+                case KER_PARABOLA: LAUNCH_CUT(KER_PARABOLA, 1); break;
+                case KER_FRACTIONAL: LAUNCH_CUT(KER_FRACTIONAL, 1); break;
+                case KER_MICRO: LAUNCH_CUT(KER_MICRO, 1); break;
+                default: LAUNCH_CUT(KER_MICRO_FIELD, 2); break;
+            }
         }
 #undef LAUNCH_CUT
+#undef LAUNCH_CUT8
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_CUT);
