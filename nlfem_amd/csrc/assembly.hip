@@ -601,6 +601,8 @@ struct PatArgs {
     const long long* tptr;
     const int* tadj;
     int EH, DH, SB;  // element hash, dof hash, sort buffer sizes (powers of two)
+    int* tmp;        // unsorted rows, RC entries each
+    int RC;
     int* rowcnt;
     const long long* rowptr;
     int* col;
@@ -608,16 +610,17 @@ struct PatArgs {
 };
 
 // Row r of the full pattern = union over elements T holding node r, and over
-// U in N(T) u N^T(T), of nodes(T) u nodes(U).  One block of 256 threads per node row.
-template <bool FILL>
+// U in N(T) u N^T(T), of nodes(T) u nodes(U).  One block of 256 threads per node row: an LDS
+// hash of the elements, then of their nodes; each new node is appended (unsorted) to the
+// row's slot of tmp (RC entries per row) and the row length to rowcnt.
 __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
     extern __shared__ int lds[];
     int* eh = lds;
     int* dh = eh + a.EH;
-    int* sb = dh + a.DH;
     __shared__ int nDof, ovf;
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     for (int i = tid; i < a.EH; i += 256) eh[i] = -1;
     for (int i = tid; i < a.DH; i += 256) dh[i] = -1;
     if (tid == 0) { nDof = 0; ovf = 0; }
@@ -639,45 +642,53 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
         }
     }
     __syncthreads();
-    for (int i = tid; i < a.EH; i += 256) {
-        int U = eh[i];
-        if (U < 0) continue;
+    int* row = a.tmp + (size_t)r * a.RC;
+    for (int i = tid; i < a.EH; i += 256) {  // EH is a multiple of 256: uniform trip count
+        const int U = eh[i];
         for (int j = 0; j < 3; j++) {
-            int d = node_of(a.tri, U, j, a.is_DG);
-            int rr = lds_insert(dh, a.DH, d);
-            if (rr < 0) ovf = 1;
-            else if (rr == 1) atomicAdd(&nDof, 1);
+            bool isnew = false;
+            int d = -1;
+            if (U >= 0) {
+                d = node_of(a.tri, U, j, a.is_DG);
+                const int rr = lds_insert(dh, a.DH, d);
+                if (rr < 0) ovf = 1;
+                isnew = rr == 1;
+            }
+            const unsigned long long m = __ballot(isnew);
+            int base = 0;
+            if (lane == 0 && m) base = atomicAdd(&nDof, __popcll(m));
+            base = __shfl(base, 0, 64);
+            if (isnew) {
+                const int q = base + __popcll(m & ((1ull << lane) - 1ull));
+                if (q < a.RC) row[q] = d;
+            }
         }
     }
     __syncthreads();
-    if (ovf) {
-        if (tid == 0) atomicOr(&a.counters[CNT_OVERFLOW], 2ull);
-        if (!FILL && tid == 0) a.rowcnt[r] = 0;
-        return;
-    }
-    const int n = nDof;
-    if (!FILL) {
-        if (tid == 0) {
+    if (tid == 0) {
+        const int n = nDof;
+        if (ovf) {
+            atomicOr(&a.counters[CNT_OVERFLOW], 2ull);
+            a.rowcnt[r] = 0;
+        } else {
+            if (n > a.RC) atomicOr(&a.counters[CNT_OVERFLOW], 4ull);
             a.rowcnt[r] = n;
             atomicMax(&a.counters[CNT_MAXROW], (unsigned long long)n);
         }
-        return;
     }
-    if (n > a.SB) {
-        if (tid == 0) atomicOr(&a.counters[CNT_OVERFLOW], 4ull);
-        return;
-    }
-    // compact the dof hash into the sort buffer, pad to a power of two, bitonic sort
-    __shared__ int fillc;
-    if (tid == 0) fillc = 0;
-    __syncthreads();
-    for (int i = tid; i < a.DH; i += 256) {
-        int d = dh[i];
-        if (d >= 0) sb[atomicAdd(&fillc, 1)] = d;
-    }
+}
+
+// Sorted column indices of each row from its unsorted slot in tmp: LDS bitonic sort.
+__global__ __launch_bounds__(256) void k_pattern_sort(PatArgs a) {
+    extern __shared__ int sb[];
+    const int r = blockIdx.x;
+    const int tid = threadIdx.x;
+    const long long o = a.rowptr[r];
+    const int n = (int)(a.rowptr[r + 1] - o);
+    const int* row = a.tmp + (size_t)r * a.RC;
     int m = 1;
     while (m < n) m <<= 1;
-    for (int i = n + tid; i < m; i += 256) sb[i] = 0x7fffffff;
+    for (int i = tid; i < m; i += 256) sb[i] = i < n ? row[i] : 0x7fffffff;
     __syncthreads();
     for (int k = 2; k <= m; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -692,7 +703,6 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
             __syncthreads();
         }
     }
-    const long long o = a.rowptr[r];
     for (int i = tid; i < n; i += 256) a.col[o + i] = sb[i];
 }
 
@@ -2712,23 +2722,30 @@ void plan_assemble(Plan& P, const Estimates& est) {
     int* d_rowcnt = nullptr;
     dalloc(d_rowcnt, (size_t)P.nNodes + 1);
     dalloc(P.d_rowptr, (size_t)P.nNodes + 1);
+    int* d_ptmp = nullptr;
+    long long rc = next_pow2((long long)(0.6 * (c.is_DG ? 4 * ecap : ecap)) + 64);
     for (int attempt = 0;; attempt++) {
         pa.EH = (int)next_pow2(2 * ecap);
         pa.DH = (int)next_pow2(c.is_DG ? 4 * ecap : ecap);
+        if (rc > pa.DH) rc = pa.DH;
         pa.SB = 0;
+        pa.RC = (int)rc;
+        dalloc(d_ptmp, (size_t)P.nNodes * (size_t)rc);
+        pa.tmp = d_ptmp;
         pa.rowcnt = d_rowcnt;
         pa.rowptr = nullptr;
         pa.col = nullptr;
         size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH);
         if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
-        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
         HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
-        k_pattern<false><<<P.nNodes, 256, ldsb, st>>>(pa);
+        k_pattern<<<P.nNodes, 256, ldsb, st>>>(pa);
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
         if (!h[CNT_OVERFLOW]) break;
-        if (attempt >= 3) throw std::length_error("pattern capacity overflow");
-        ecap *= 2;
+        if (attempt >= 4) throw std::length_error("pattern capacity overflow");
+        if (h[CNT_OVERFLOW] & 2ull) ecap *= 2;   // a hash table filled up
+        else rc *= 2;                            // a row outgrew its slot
     }
     P.maxRow = (int)h[CNT_MAXROW];
     scan_exclusive<int>(d_rowcnt, P.d_rowptr, P.nNodes, scan_scratch(P), st);
@@ -2740,12 +2757,13 @@ void plan_assemble(Plan& P, const Estimates& est) {
     pa.rowptr = P.d_rowptr;
     pa.col = P.d_col;
     {
-        size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH + pa.SB);
-        if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
-        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
-        k_pattern<true><<<P.nNodes, 256, ldsb, st>>>(pa);
+        size_t ldsb = sizeof(int) * (size_t)pa.SB;
+        if (ldsb > 160 * 1024) throw std::length_error("pattern row too long for the LDS sort");
+        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern_sort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+        k_pattern_sort<<<P.nNodes, 256, ldsb, st>>>(pa);
         HIPCHECK(hipGetLastError());
     }
+    (void)hipFree(d_ptmp);
     ev_end(P, PH_PATTERN);
 
     // ---------------------------------------------------------------- positions
