@@ -202,15 +202,18 @@ NLFEM_DEV int retriangulate(double xc0, double xc1, const double* TE, double sqd
         const double bb = b0 * b0 + b1 * b1;
         const double v = ab * ab - (aa - sqdelta) * bb;
         if (v >= 0) {
-            const double term1 = -ab / bb;
-            const double term2 = sqrt(v) / bb;
+            // the three divisions by bb use one correctly rounded reciprocal and a Markstein
+            // correction each, which rounds as the division (bb > 0: elements are not degenerate)
+            const double rbb = 1.0 / bb;
+            const double term1 = div_by_recip(-ab, bb, rbb);
+            const double term2 = div_by_recip(sqrt(v), bb, rbb);
             double lam1, lam2;
             if (term1 > 0) {
                 lam1 = term1 + term2;
-                lam2 = 1 / lam1 * (aa - sqdelta) / bb;
+                lam2 = div_by_recip(1 / lam1 * (aa - sqdelta), bb, rbb);
             } else {
                 lam2 = term1 - term2;
-                lam1 = 1 / lam2 * (aa - sqdelta) / bb;
+                lam1 = div_by_recip(1 / lam2 * (aa - sqdelta), bb, rbb);
             }
             if ((0 <= lam1) && (lam1 <= 1)) {
                 const double y0 = lam1 * b0 + Q0, y1 = lam1 * b1 + Q1;
