@@ -242,11 +242,17 @@ __global__ void k_dual(int nE, const int* tri, const long long* vptr, const int*
     }
 }
 
-__global__ void k_adjpad(int nE, const long long* xadj, const int* adj, int W, int* adjp) {
+// Padded rows of the dual graph (W entries, -1 beyond the degree); also the graph's index
+// bandwidth max |e - adj(e)| into *bw.
+__global__ void k_adjpad(int nE, const long long* xadj, const int* adj, int W, int* adjp, unsigned long long* bw) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nE) return;
     const long long b = xadj[e], d = xadj[e + 1] - b;
-    for (int j = 0; j < W; j++) adjp[(size_t)e * W + j] = j < d ? adj[b + j] : -1;
+    int m = 0;
+    for (long long j = 0; j < d; j++) m = max(m, abs(adj[b + j] - e));
+    if (adjp)
+        for (int j = 0; j < W; j++) adjp[(size_t)e * W + j] = j < d ? adj[b + j] : -1;
+    atomicMax(bw, (unsigned long long)m);
 }
 
 // ======================================================================= BFS
@@ -260,7 +266,8 @@ struct BfsArgs {
     QuadArgs q;
     double sqdelta, sq_in, sq_cut, sq_far;
     int caps, remote, level0, fullCC;
-    int cap, H, F;
+    int cap, H, F;    // list capacity; visited-set words (hash slots or bitmap words); level buffers
+    int whalf;        // bitmap window: element e of outer element aT is bit e - (aT - whalf)
     uint32_t* nbr;
     int* cntF;
     int* cntB;
@@ -367,11 +374,31 @@ NLFEM_DEV int classify_full(const BfsArgs& a, const OuterPts& op, const ElemRec&
     return a.fullCC ? (int)CLS_CUT : -1;
 }
 
+// Padded dual-graph rows of up to 4 * BFS_W4MAX entries are loaded at once; longer rows use the CSR.
+constexpr int BFS_W4MAX = 4;
+
 // Position of this lane among the lanes of the wavefront for which p holds; total = their number.
 NLFEM_DEV int wave_rank(bool p, int& total) {
     const unsigned long long m = __ballot(p);
     total = __popcll(m);
     return __popcll(m & ((1ull << __lane_id()) - 1ull));
+}
+
+// Visited set of one outer element: an LDS hash of element indices (BM = false), or a bitmap
+// over the index window [aT - whalf, aT + whalf) when the dual graph's index bandwidth makes
+// that window small (BM = true; an index outside it reports overflow and the host falls back
+// to the hash).  Returns 1 when e is new, 0 when already visited, -1 on overflow.
+template <bool BM>
+NLFEM_DEV int visit_insert(int* vs, const BfsArgs& a, int aT, int e) {
+    if constexpr (BM) {
+        const int idx = e - (aT - a.whalf);
+        if ((unsigned)idx >= (unsigned)(32 * a.H)) return -1;
+        const unsigned bit = 1u << (idx & 31);
+        const unsigned old = atomicOr((unsigned*)&vs[idx >> 5], bit);
+        return (old & bit) ? 0 : 1;
+    } else {
+        return lds_insert(vs, a.H, e);
+    }
 }
 
 // One wavefront (one block of 64 lanes) per outer element.  Equivalent to the reference's
@@ -381,6 +408,7 @@ NLFEM_DEV int wave_rank(bool p, int& total) {
 // All counters are wavefront-uniform registers, appended to with ballot ranks (no LDS atomics);
 // the candidates that only a retriangulation can settle are compacted in place and tested
 // densely after the cheap tests of the level.
+template <bool BM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_bfs(BfsArgs a) {
     extern __shared__ int lds[];
     int* hash = lds;
@@ -395,11 +423,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (lane == 0) { a.cntF[aT] = 0; a.cntB[aT] = 0; }
         return;
     }
-    for (int i = lane; i < a.H; i += 64) hash[i] = -1;
+    for (int i = lane; i < a.H; i += 64) hash[i] = BM ? 0 : -1;
     if (lane == 0) outer_points(a, A, op);
     __syncthreads();
     int nTest = 0, nNext = 0, nTot = 0, nF = 0, nB = 0;
-    bool ovf = false;
+    bool ovf = false, wovf = false;  // wovf: an index fell outside the bitmap window
     unsigned long long cProbe = 0, cClose = 0, cInt = 0, cCut = 0, cBary = 0;
     // level 0 candidates: aT itself, then its dual-graph neighbours
     const long long x0 = a.xadj[aT];
@@ -407,15 +435,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (1 + deg > a.F) {
         ovf = true;
     } else {
-        if (lane == 0) { lds_insert(hash, a.H, aT); test[0] = aT; }
+        if (lane == 0) { visit_insert<BM>(hash, a, aT, aT); test[0] = aT; }
         __syncthreads();
         bool lo = false;
         for (int j = lane; j < deg; j += 64) {
             const int b = a.adj[x0 + j];
-            if (lds_insert(hash, a.H, b) < 0) lo = true;
+            if (visit_insert<BM>(hash, a, aT, b) < 0) lo = true;
             test[1 + j] = b;
         }
         ovf = __any(lo);
+        wovf = BM && ovf;
         nTest = 1 + deg;
     }
     __syncthreads();
@@ -497,17 +526,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         // discover the next level: unvisited neighbours of the expanded elements
         nTest = 0;
         bool lovf = false;
-        auto visit = [&](int e) {  // wavefront-uniform call; e < 0 for no candidate
-            bool isnew = false;
-            if (e >= 0) {
-                const int r = lds_insert(hash, a.H, e);
-                if (r < 0) lovf = true;
-                isnew = r == 1;
-            }
+        auto append = [&](int e, bool isnew) {  // wavefront-uniform call
             int nt;
             const int rt = wave_rank(isnew, nt);
             if (isnew && nTest + rt < a.F) test[nTest + rt] = e;
             nTest += nt;
+        };
+        auto visit = [&](int e) {  // wavefront-uniform call; e < 0 for no candidate
+            bool isnew = false;
+            if (e >= 0) {
+                const int r = visit_insert<BM>(hash, a, aT, e);
+                if (r < 0) lovf = true;
+                isnew = r == 1;
+            }
+            append(e, isnew);
         };
         for (int r0 = 0; r0 < nNext; r0 += 64) {
             const int j = r0 + lane;
@@ -515,17 +547,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             if (a.W4 > 0) {
                 // padded row: all loads issued before the first insert
                 const int4* row = reinterpret_cast<const int4*>(a.adjp) + (size_t)(s < 0 ? 0 : s) * a.W4;
-                int4 v[8];
+                int4 v[BFS_W4MAX];
 #pragma unroll
-                for (int c = 0; c < 8; c++)
+                for (int c = 0; c < BFS_W4MAX; c++)
                     if (c < a.W4) v[c] = s >= 0 ? row[c] : make_int4(-1, -1, -1, -1);
+                if constexpr (BM) {
+                    // all bitmap updates of the row are issued before any result is used
+                    int ev[4 * BFS_W4MAX];
+                    unsigned old[4 * BFS_W4MAX];
 #pragma unroll
-                for (int c = 0; c < 8; c++) {
-                    if (c >= a.W4) break;
-                    visit(v[c].x);
-                    visit(v[c].y);
-                    visit(v[c].z);
-                    visit(v[c].w);
+                    for (int c = 0; c < BFS_W4MAX; c++) {
+                        ev[4 * c] = v[c].x; ev[4 * c + 1] = v[c].y; ev[4 * c + 2] = v[c].z; ev[4 * c + 3] = v[c].w;
+                    }
+                    const int base = aT - a.whalf;
+#pragma unroll
+                    for (int q = 0; q < 4 * BFS_W4MAX; q++) {
+                        old[q] = 0xffffffffu;
+                        if (q < 4 * a.W4 && ev[q] >= 0) {
+                            const int idx = ev[q] - base;
+                            if ((unsigned)idx >= (unsigned)(32 * a.H)) lovf = true;
+                            else old[q] = atomicOr((unsigned*)&hash[idx >> 5], 1u << (idx & 31));
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4 * BFS_W4MAX; q++) {
+                        if (q >= 4 * a.W4) break;
+                        const int idx = ev[q] - base;
+                        const bool isnew = ev[q] >= 0 && !((old[q] >> (idx & 31)) & 1u);
+                        append(ev[q], isnew);
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < BFS_W4MAX; c++) {
+                        if (c >= a.W4) break;
+                        visit(v[c].x);
+                        visit(v[c].y);
+                        visit(v[c].z);
+                        visit(v[c].w);
+                    }
                 }
             } else {
                 const long long p0 = s >= 0 ? a.xadj[s] : 0;
@@ -535,7 +594,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 for (int p = 0; p < mlen; p++) visit(p < len ? a.adj[p0 + p] : -1);
             }
         }
-        ovf = __any(lovf) || nTest > a.F;
+        const bool al = __any(lovf);
+        if (BM && al) wovf = true;
+        ovf = al || nTest > a.F;
         nNext = 0;
         __syncthreads();
         level++;
@@ -550,7 +611,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
     if (lane == 0) {
         if (ovf) {
-            atomicOr(&a.counters[CNT_OVERFLOW], 1ull);
+            atomicOr(&a.counters[CNT_OVERFLOW], wovf ? 2ull : 1ull);
             a.cntF[aT] = 0;
             a.cntB[aT] = 0;
         } else {
@@ -2614,12 +2675,14 @@ void plan_assemble(Plan& P, const Estimates& est) {
     read_counters(P, h);
     const int maxdeg = (int)h[CNT_MAXT];
     int W4 = (maxdeg + 3) / 4;
-    if (W4 > 8) W4 = 0;  // very irregular meshes: BFS walks the CSR rows
-    if (W4 > 0) {
-        dalloc(P.d_adjp, (size_t)c.nE * 4 * W4);
-        k_adjpad<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_xadj, P.d_adj, 4 * W4, P.d_adjp);
-        HIPCHECK(hipGetLastError());
-    }
+    if (W4 > BFS_W4MAX) W4 = 0;  // irregular meshes: BFS walks the CSR rows
+    if (W4 > 0) dalloc(P.d_adjp, (size_t)c.nE * 4 * W4);
+    HIPCHECK(hipMemsetAsync(P.d_counters + CNT_MAXROW, 0, sizeof(unsigned long long), st));
+    k_adjpad<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_xadj, P.d_adj, 4 * W4, W4 > 0 ? P.d_adjp : nullptr,
+                                               P.d_counters + CNT_MAXROW);
+    HIPCHECK(hipGetLastError());
+    read_counters(P, h);
+    const long long bandwidth = (long long)h[CNT_MAXROW];
     ev_end(P, PH_DUAL);
 
     // ---------------------------------------------------------------- BFS
@@ -2651,17 +2714,28 @@ void plan_assemble(Plan& P, const Estimates& est) {
     long long hvis = c.fullCC ? (long long)c.nE : (long long)(1.05 * area_est) + 64;
     long long fvis = (long long)(8.0 * std::sqrt(area_est)) + 64;
     if (P.bfs_visit > 0) { visit = P.bfs_visit; hvis = P.bfs_hvis; fvis = P.bfs_fvis; }  // sizes that worked before
+    // Bitmap visited set: after L levels the BFS has moved at most L * bandwidth indices from
+    // aT, and it needs at most about (delta + 2 hmax) / hmin levels (hmin: smallest element
+    // height), so a window of +-bandwidth * (levels + 3) covers the search.  An index outside
+    // the window is reported and the BFS reruns with the hash.
+    const double hmin = 2.0 * est.amin / est.hmax;
+    const long long levels = (long long)std::ceil((delta + 2.0 * est.hmax) / hmin) + 3;
+    const long long whalf = bandwidth * levels + 1;
+    const long long bm_words = (2 * whalf + 31) / 32;
+    bool use_bm = !c.fullCC && W4 > 0 && bm_words <= 8192 && !P.bfs_nobm;
     for (int attempt = 0;; attempt++) {
         long long cap = std::max<long long>(visit, 16);
         long long H = next_pow2(std::min<long long>(std::max<long long>(hvis, 64), 2 * cap));
         long long F = std::min<long long>(next_pow2(std::max<long long>(fvis, 256)), next_pow2(cap));
+        if (use_bm) H = bm_words;
         size_t ldsb = sizeof(int) * (size_t)(H + 2 * F);
         if (ldsb > 160 * 1024) {
             // too large for LDS: fall back to the largest table that fits and rely on overflow detection
-            H = 16384;
+            H = use_bm ? H : 16384;
             F = std::min<long long>(F, 8192);
             ldsb = sizeof(int) * (size_t)(H + 2 * F);
         }
+        ba.whalf = (int)whalf;
         P.cap = (int)cap;
         dalloc(P.d_nbr, (size_t)c.nE * (size_t)cap);
         ba.cap = (int)cap; ba.H = (int)H; ba.F = (int)F;
@@ -2669,8 +2743,13 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ba.counters = P.d_counters;
         HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
         ev_begin(P, PH_BFS);
-        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
-        k_bfs<<<c.nE, 64, ldsb, st>>>(ba);
+        if (use_bm) {
+            if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+            k_bfs<true><<<c.nE, 64, ldsb, st>>>(ba);
+        } else {
+            if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+            k_bfs<false><<<c.nE, 64, ldsb, st>>>(ba);
+        }
         HIPCHECK(hipGetLastError());
         ev_end(P, PH_BFS);
         read_counters(P, h);
@@ -2678,6 +2757,11 @@ void plan_assemble(Plan& P, const Estimates& est) {
         if (!h[CNT_OVERFLOW]) {
             P.bfs_visit = visit; P.bfs_hvis = hvis; P.bfs_fvis = fvis;
             break;
+        }
+        if (use_bm && (h[CNT_OVERFLOW] & 2ull)) {  // left the bitmap window: hash from now on
+            use_bm = false;
+            P.bfs_nobm = true;
+            continue;
         }
         if (attempt >= 3 || (H >= 16384 && ldsb > 64 * 1024 && visit >= c.nE))
             throw std::length_error("BFS capacity overflow: interaction horizon too large for this build");

@@ -121,6 +121,7 @@ struct Plan {
     size_t scan_tmp_bytes = 0;
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
+    bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
 
     Stats stats;
     bool assembled = false;

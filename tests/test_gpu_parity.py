@@ -164,3 +164,14 @@ def test_invalid_configuration_raises():
     bad["approxBalls"] = {"method": "fractional"}
     with pytest.raises(nlfem.NlfemError):
         nlfem.stiffnessMatrix(mesh, kernel, bad)
+
+
+def test_parity_shuffled_element_numbering():
+    """Elements in random order: the dual graph's index bandwidth is large, so the BFS uses
+    the hashed visited set instead of the index-window bitmap."""
+    from nlfem_amd.mesh import unit_square, mesh_dict
+    from nlfem_amd.config import make_configuration
+    m = unit_square(48, 0.1, jitter=0.1, seed=9)
+    perm = np.random.default_rng(3).permutation(m["nE"])
+    ms = mesh_dict(m["elements"][perm], m["elementLabels"][perm], m["vertices"], m["vertexLabels"])
+    _check_case("shuffled", ms, {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration())
