@@ -664,6 +664,8 @@ struct PatArgs {
     int EH, DH, SB;  // element hash, dof hash, sort buffer sizes (powers of two)
     int* tmp;        // unsorted rows, RC entries each
     int RC;
+    int whalf;       // bitmap element set: element U of row r is bit U - (T0 - whalf), T0 = first element at r
+    int EL;          // bitmap mode: capacity of the compacted element list (a multiple of 256)
     int* rowcnt;
     const long long* rowptr;
     int* col;
@@ -674,6 +676,9 @@ struct PatArgs {
 // U in N(T) u N^T(T), of nodes(T) u nodes(U).  One block of 256 threads per node row: an LDS
 // hash of the elements, then of their nodes; each new node is appended (unsorted) to the
 // row's slot of tmp (RC entries per row) and the row length to rowcnt.
+// BM: the element set is a bitmap over an index window around the row's first element
+// (EH words; an element outside it reports overflow 8 and the host reruns with the hash).
+template <bool BM>
 __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
     extern __shared__ int lds[];
     int* eh = lds;
@@ -682,30 +687,76 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    for (int i = tid; i < a.EH; i += 256) eh[i] = -1;
+    for (int i = tid; i < a.EH; i += 256) eh[i] = BM ? 0 : -1;
     for (int i = tid; i < a.DH; i += 256) dh[i] = -1;
     if (tid == 0) { nDof = 0; ovf = 0; }
     __syncthreads();
     long long t0, t1;
     if (a.is_DG) { t0 = r / 3; t1 = t0 + 1; }
     else { t0 = a.vptr[r]; t1 = a.vptr[r + 1]; }
+    const int base = BM ? (a.is_DG ? (int)t0 : a.vel[t0]) - a.whalf : 0;
+    auto einsert = [&](int U) {
+        if constexpr (BM) {
+            const int idx = U - base;
+            if ((unsigned)idx >= (unsigned)(32 * a.EH)) ovf = 8;
+            else atomicOr((unsigned*)&eh[idx >> 5], 1u << (idx & 31));
+        } else {
+            if (lds_insert(eh, a.EH, U) < 0) ovf = 1;
+        }
+    };
     for (long long ti = t0; ti < t1; ti++) {
         const int T = a.is_DG ? (int)ti : a.vel[ti];
         const int nF = a.cntF[T], n = nF + a.cntB[T];
         const long long tb = a.tptr[T], te = a.tptr[T + 1];
         const int nt = (int)(te - tb);
-        if (n + nt > 0 && tid == 0) { if (lds_insert(eh, a.EH, T) < 0) ovf = 1; }
+        if (n + nt > 0 && tid == 0) einsert(T);
         for (int p = tid; p < n + nt; p += 256) {
             int U;
             if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
             else U = (int)((uint32_t)a.tadj[tb + (p - n)] & IDX_MASK);
-            if (lds_insert(eh, a.EH, U) < 0) ovf = 1;
+            einsert(U);
         }
     }
     __syncthreads();
     int* row = a.tmp + (size_t)r * a.RC;
-    for (int i = tid; i < a.EH; i += 256) {  // EH is a multiple of 256: uniform trip count
-        const int U = eh[i];
+    int nslots = a.EH;
+    const int* elist = eh;
+    if constexpr (BM) {
+        // compact the set bits into a list of elements (block-wide exclusive scan of the counts)
+        int* el = dh + a.DH;
+        __shared__ int wtot[4];
+        __shared__ int ntotal;
+        int cnt = 0;
+        for (int i = tid; i < a.EH; i += 256) cnt += __popc((unsigned)eh[i]);
+        int v = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(v, off, 64);
+            if (lane >= off) v += y;
+        }
+        if (lane == 63) wtot[tid >> 6] = v;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < (tid >> 6); w++) woff += wtot[w];
+        if (tid == 255) ntotal = woff + v;
+        int o = woff + v - cnt;
+        if (woff + v > a.EL) ovf = 1;
+        else
+            for (int i = tid; i < a.EH; i += 256) {
+                unsigned bits = (unsigned)eh[i];
+                while (bits) {
+                    const int b = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    el[o++] = base + 32 * i + b;
+                }
+            }
+        __syncthreads();
+        nslots = (min(ntotal, a.EL) + 255) & ~255;
+        for (int i = ntotal + tid; i < nslots; i += 256) el[i] = -1;
+        __syncthreads();
+        elist = el;
+    }
+    for (int i = tid; i < nslots; i += 256) {  // a multiple of 256: uniform trip count
+        const int U = elist[i];
         for (int j = 0; j < 3; j++) {
             bool isnew = false;
             int d = -1;
@@ -729,7 +780,7 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
     if (tid == 0) {
         const int n = nDof;
         if (ovf) {
-            atomicOr(&a.counters[CNT_OVERFLOW], 2ull);
+            atomicOr(&a.counters[CNT_OVERFLOW], ovf == 8 ? 8ull : 2ull);
             a.rowcnt[r] = 0;
         } else {
             if (n > a.RC) atomicOr(&a.counters[CNT_OVERFLOW], 4ull);
@@ -2808,9 +2859,16 @@ void plan_assemble(Plan& P, const Estimates& est) {
     dalloc(P.d_rowptr, (size_t)P.nNodes + 1);
     int* d_ptmp = nullptr;
     long long rc = next_pow2((long long)(0.6 * (c.is_DG ? 4 * ecap : ecap)) + 64);
+    // element set as a bitmap: the row's elements lie within levels + 2 dual-graph steps of
+    // the row's first element, each step moving at most `bandwidth` indices
+    const long long pwhalf = bandwidth * (levels + 2) + 1;
+    const long long pwords = next_pow2((2 * pwhalf + 31) / 32 + 8);  // 32*pwords is a multiple of 256
+    bool pat_bm = !c.fullCC && !P.pat_nobm && pwords <= 4096;
+    pa.whalf = (int)pwhalf;
     for (int attempt = 0;; attempt++) {
-        pa.EH = (int)next_pow2(2 * ecap);
+        pa.EH = pat_bm ? (int)pwords : (int)next_pow2(2 * ecap);
         pa.DH = (int)next_pow2(c.is_DG ? 4 * ecap : ecap);
+        pa.EL = pat_bm ? (int)((ecap + 255) & ~255LL) : 0;
         if (rc > pa.DH) rc = pa.DH;
         pa.SB = 0;
         pa.RC = (int)rc;
@@ -2819,15 +2877,25 @@ void plan_assemble(Plan& P, const Estimates& est) {
         pa.rowcnt = d_rowcnt;
         pa.rowptr = nullptr;
         pa.col = nullptr;
-        size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH);
+        size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH + pa.EL);
         if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
-        if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
         HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
-        k_pattern<<<P.nNodes, 256, ldsb, st>>>(pa);
+        if (pat_bm) {
+            if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+            k_pattern<true><<<P.nNodes, 256, ldsb, st>>>(pa);
+        } else {
+            if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+            k_pattern<false><<<P.nNodes, 256, ldsb, st>>>(pa);
+        }
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
         if (!h[CNT_OVERFLOW]) break;
-        if (attempt >= 4) throw std::length_error("pattern capacity overflow");
+        if (attempt >= 5) throw std::length_error("pattern capacity overflow");
+        if (pat_bm && (h[CNT_OVERFLOW] & 8ull)) {  // left the bitmap window: hash from now on
+            pat_bm = false;
+            P.pat_nobm = true;
+            continue;
+        }
         if (h[CNT_OVERFLOW] & 2ull) ecap *= 2;   // a hash table filled up
         else rc *= 2;                            // a row outgrew its slot
     }

@@ -122,6 +122,7 @@ struct Plan {
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
+    bool pat_nobm = false;  // the same for the pattern's element set
 
     Stats stats;
     bool assembled = false;
