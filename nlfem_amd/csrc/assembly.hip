@@ -1375,12 +1375,10 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
             const int U = (int)(e & IDX_MASK);
             const double w = a.kp.c * A.absDet * a.elem[U].absDet;
             sumB += a.elem[U].absDet;
-            for (int j = 0; j < 3; j++) {
+            for (int j = 0; j < 3; j++) {  // column U_j collects w q1_j; row T_i takes p1_i of it below
                 const int s = lds_find(hk, a.HN, node_of(a.tri, U, j, a.is_DG));
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
-                const double wq = w * a.q1[j];
-                for (int i = 0; i < 3; i++)
-                    if (labA[i]) atomicAdd(&acc[s * 3 + i], -(wq * a.p1[i]));
+                atomicAdd(&acc[s * 3 + 0], w * a.q1[j]);
             }
         }
         const long long tb = a.tptr[T0], te = a.tptr[T0 + 1];
@@ -1391,13 +1389,17 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
             const double absU = a.elem[U].absDet;
             const double w = a.kp.c * absU * A.absDet;
             sumU += absU;
-            for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i): outer U, inner T
+            for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i): column U_i collects w p1_i
                 const int s = lds_find(hk, a.HN, node_of(a.tri, U, i, a.is_DG));
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
-                const double wp = w * a.p1[i];
-                for (int j = 0; j < 3; j++)
-                    if (labA[j]) atomicAdd(&acc[s * 3 + j], -(wp * a.q1[j]));
+                atomicAdd(&acc[s * 3 + 1], w * a.p1[i]);
             }
+        }
+        __syncthreads();
+        // rows of T: acc[s][i] = -(p1_i Wout_s + q1_i Win_s) for assembled rows
+        for (int sl = tid; sl < a.HN; sl += 256) {
+            const double wo = acc[sl * 3 + 0], wi = acc[sl * 3 + 1];
+            for (int i = 0; i < 3; i++) acc[sl * 3 + i] = labA[i] ? -(a.p1[i] * wo + a.q1[i] * wi) : 0.0;
         }
     }
     for (int p = tid; p < (FACT ? 0 : nF); p += 256) {
