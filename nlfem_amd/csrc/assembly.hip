@@ -1632,51 +1632,65 @@ struct PointCollector {
 // termNonloc and termLocalPrime into pacc.  termLocal only goes to the outer element's own
 // rows, so tls[k*OO + o] collects innerLocal over all pairs of the block per outer point.
 // Inner sums of one outer point for the constant kernel, accumulated from the fan triangles
-// (R0, R_j, R_j+1) as the retriangulation emits them.  With the basis of bT affine on each fan
-// triangle, the inner rule's sums of 1, psi_b and psi_a psi_b over the triangle's
-// pulled-back points equal the rule's moments (q0, qm) applied to the basis values at the
-// triangle's corners, so each corner is pulled back once (the reference's pivoted toRef)
-// instead of each inner point; the results agree with the point-by-point sums to rounding.
+// (R0, R_j, R_j+1) as the retriangulation emits them.  The basis of bT is affine, psi_b(y) =
+// psi_b(R0) + grad psi_b . (y - R0), so on a fan triangle the inner rule's sums of 1, psi_b
+// and psi_a psi_b over its pulled-back points are linear in the rule's moments (q0, qm) of
+// the corner offsets d = R_j - R0.  Per fan triangle only the geometric moments
+//   m0 = sum |det| q0-part,  m1_c = sum |det| (d1_c mx + d2_c my),
+//   m2_cd = sum |det| (d1_c d1_d mxx + (d1_c d2_d + d2_c d1_d) mxy + d2_c d2_d myy)
+// are accumulated (offsets from the fan apex, physical coordinates); finish() applies the
+// basis once per item: psi at the apex from the reference's pivoted toRef, the gradients
+// from the inverse Jacobian of bT.  The results agree with the point-by-point sums of
+// integrate_retriangulate (src/integration.cpp:418-506) to rounding.
 struct ConstantItem {
     const NumArgs& a;
     const PivotLU& lu;
+    const double* E;  // vertices of bT
     double inL = 0, inN[3] = {0, 0, 0}, sLP[6] = {0, 0, 0, 0, 0, 0};
-    double g0[3], gp[3];
+    double m0 = 0, m1[2] = {0, 0}, m2[3] = {0, 0, 0};
+    double f0 = 0, f1 = 0, e0 = 0, e1 = 0;  // apex R0, offset of the previous corner
     int nt = 0;
-    NLFEM_DEV ConstantItem(const NumArgs& a_, const PivotLU& lu_) : a(a_), lu(lu_) {}
-    NLFEM_DEV void basis(double y0, double y1, double* g) const {
-        double p0, p1;
-        lu.solve(y0, y1, p0, p1);
-        g[0] = 1 - p0 - p1;
-        g[1] = p0;
-        g[2] = p1;
-    }
+    NLFEM_DEV ConstantItem(const NumArgs& a_, const PivotLU& lu_, const double* E_) : a(a_), lu(lu_), E(E_) {}
     NLFEM_DEV void operator()(double r0, double r1, double s0, double s1, double t0, double t1) {
 #pragma clang fp contract(fast)
         if (nt == 0) {
-            basis(r0, r1, g0);
-            basis(s0, s1, gp);
+            f0 = r0; f1 = r1;
+            e0 = s0 - r0; e1 = s1 - r1;
         }
         nt++;
-        double gn[3];
-        basis(t0, t1, gn);
-        const double rT[6] = {r0, r1, s0, s1, t0, t1};
-        const double w = a.kp.c * absDet(rT);
-        const double q0 = a.q0, mx = a.qm[0], my = a.qm[1], mxx = a.qm[2], mxy = a.qm[3], myy = a.qm[4];
-        double d1[3], d2[3];
-        for (int b = 0; b < 3; b++) {
-            d1[b] = gp[b] - g0[b];
-            d2[b] = gn[b] - g0[b];
-            gp[b] = gn[b];
-        }
-        inL += w * q0;
-        for (int b = 0; b < 3; b++) inN[b] += w * (g0[b] * q0 + d1[b] * mx + d2[b] * my);
+        const double d0 = t0 - f0, d1 = t1 - f1;
+        const double D = absolute(e0 * d1 - d0 * e1);
+        const double mx = a.qm[0], my = a.qm[1], mxx = a.qm[2], mxy = a.qm[3], myy = a.qm[4];
+        const double u0 = e0 * mxx + d0 * mxy, u1 = e1 * mxx + d1 * mxy;
+        const double v0 = e0 * mxy + d0 * myy, v1 = e1 * mxy + d1 * myy;
+        m0 += D;
+        m1[0] += D * (e0 * mx + d0 * my);
+        m1[1] += D * (e1 * mx + d1 * my);
+        m2[0] += D * (e0 * u0 + d0 * v0);
+        m2[1] += D * (e1 * u0 + d1 * v0);
+        m2[2] += D * (e1 * u1 + d1 * v1);
+        e0 = d0; e1 = d1;
+    }
+    NLFEM_DEV void finish() {
+#pragma clang fp contract(fast)
+        if (nt == 0) return;
+        double p0, p1;
+        lu.solve(f0, f1, p0, p1);
+        const double g[3] = {1 - p0 - p1, p0, p1};
+        const InvMap im = invmap(E);
+        // gradients: psi_1 = p0, psi_2 = p1, psi_0 = 1 - p0 - p1
+        const double G[3][2] = {{-(im.i00 + im.i10), -(im.i01 + im.i11)}, {im.i00, im.i01}, {im.i10, im.i11}};
+        const double c = a.kp.c, cm0 = c * (a.q0 * m0);
+        double A[3];
+        for (int b = 0; b < 3; b++) A[b] = G[b][0] * m1[0] + G[b][1] * m1[1];
+        inL = cm0;
+        for (int b = 0; b < 3; b++) inN[b] = g[b] * cm0 + c * A[b];
         int m = 0;
-        for (int u = 0; u < 3; u++)
+        for (int u = 0; u < 3; u++) {
+            const double h0 = G[u][0] * m2[0] + G[u][1] * m2[1], h1 = G[u][0] * m2[1] + G[u][1] * m2[2];
             for (int v = u; v < 3; v++, m++)
-                sLP[m] += w * (g0[u] * g0[v] * q0 + (g0[u] * d1[v] + d1[u] * g0[v]) * mx +
-                               (g0[u] * d2[v] + d2[u] * g0[v]) * my + d1[u] * d1[v] * mxx +
-                               (d1[u] * d2[v] + d2[u] * d1[v]) * mxy + d2[u] * d2[v] * myy);
+                sLP[m] = g[u] * g[v] * cm0 + c * (g[u] * A[v] + g[v] * A[u] + h0 * G[v][0] + h1 * G[v][1]);
+        }
     }
 };
 
@@ -1780,8 +1794,9 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
                                           seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
             if (!(dmin > a.sq_far)) {
                 if constexpr (KER == KER_CONSTANT && OD == 1) {  // integrated here: no fan triangles for phase B
-                    ConstantItem ci(a, lus[p]);
+                    ConstantItem ci(a, lus[p], B.E);
                     retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, ci);
+                    ci.finish();
                     mytri += ci.nt;
                     double* ia = iacc + tid * 10;
                     ia[0] = ci.inL;
@@ -1919,8 +1934,9 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
 template <int KER, typename T>
 __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     extern __shared__ int ldsc8[];
-    int* hk = ldsc8;      // [HN]
-    int* hv = hk + a.HN;  // [HN][3]
+    int* hk = ldsc8;                          // [HN] column nodes of aT's rows
+    int* hv = hk + a.HN;                      // [HN][3] their offsets in rows aAdx[0..2]
+    double* acc = (double*)(hv + 3 * a.HN);   // [HN][3] termNonloc (i, node) summed over pairs
     __shared__ double sred[4][8];
     const int aT = blockIdx.x;
     const int tid = threadIdx.x;
@@ -1937,13 +1953,17 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         rA[j] = node_of(a.tri, aT, j, a.is_DG);
         labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
     }
-    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    for (int i = tid; i < a.HN; i += 256) {
+        hk[i] = -1;
+        acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0;
+    }
     __syncthreads();
+    long long rp[3];
     for (int j = 0; j < 3; j++) {
-        const long long r0 = a.rowptr[rA[j]];
-        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        rp[j] = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - rp[j]);
         for (int i = tid; i < len; i += 256) {
-            int sl = lds_slot(hk, a.HN, a.col[r0 + i]);
+            int sl = lds_slot(hk, a.HN, a.col[rp[j] + i]);
             if (sl >= 0) hv[3 * sl + j] = i;
             else atomicOr(&a.counters[CNT_ERROR], 16ull);
         }
@@ -1972,8 +1992,9 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
             if (!(dmin > a.sq_far)) {
                 const PivotLU lu = pivot_lu(B.E);
                 if constexpr (KER == KER_CONSTANT) {
-                    ConstantItem ci(a, lu);
+                    ConstantItem ci(a, lu, B.E);
                     retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, ci);
+                    ci.finish();
                     mytri += ci.nt;
                     inL = ci.inL;
                     for (int q = 0; q < 3; q++) inN[q] = ci.inN[q];
@@ -2016,19 +2037,21 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
                 tN[e] += shfl_xor_d(tN[e], m);
                 tLP[e] += shfl_xor_d(tLP[e], m);
             }
-        // scatter: lane k takes entry (i, j) = k (and lane 0 also entry 8)
+        // -termNonloc (i, j) goes to the entries (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]):
+        // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
+        // the matrix once per block below
+        if (k < 3) {
+            const int sl = lds_find(hk, a.HN, node_of(a.tri, b, k, a.is_DG));
+            if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
+            else
+                for (int i = 0; i < 3; i++)
+                    atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
+        }
+        // termLocalPrime: lane k takes entry (i, j) = k (and lane 0 also entry 8)
         for (int e = k; e < 9; e += 8) {
             const int i = e / 3, j = e % 3;
-            const int rBj = node_of(a.tri, b, j, a.is_DG);
-            const bool labBj = row_labelled(a.vlabel, rBj, a.is_DG);
-            const int sl = lds_find(hk, a.HN, rBj);
-            if (sl < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
-            const long long pos = a.rowptr[rA[i]] + hv[3 * sl + i];  // entry (aAdx[i], bAdx[j])
-            if (labA[i]) atomicAdd(&a.val[pos], -tN[e]);
-            if (labBj) {
-                atomicAdd(&a.val[a.tpos[pos]], -tN[e]);
+            if (row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG))
                 atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], tLP[e]);  // (bAdx[j], bAdx[i])
-            }
         }
     }
     // termLocal[i][j] = sum_k |aT| dx_k psi_i(x_k) psi_j(x_k) innerLocal_k: sum over the lanes of
@@ -2039,6 +2062,18 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
     if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
     __syncthreads();
+    for (int sl = tid; sl < a.HN; sl += 256) {
+        const int node = hk[sl];
+        if (node < 0) continue;
+        const bool lab = row_labelled(a.vlabel, node, a.is_DG);
+        for (int i = 0; i < 3; i++) {
+            const double v = acc[3 * sl + i];
+            if (v == 0.0) continue;  // node not touched (its offsets may be unset)
+            const long long pos = rp[i] + hv[3 * sl + i];
+            if (labA[i]) atomicAdd(&a.val[pos], -v);
+            if (lab) atomicAdd(&a.val[a.tpos[pos]], -v);
+        }
+    }
     if (tid < 9) {
         const int i = tid / 3, j = tid % 3;
         if (labA[i]) {
@@ -3085,7 +3120,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     } while (0)
 #define LAUNCH_CUT8(KER)                                                                                  \
     do {                                                                                                  \
-        const size_t l8 = (size_t)nc.HN * 16;                                                             \
+        const size_t l8 = (size_t)nc.HN * 40;                                                             \
         if (c.precision == 1) {                                                                           \
             if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
             k_cut8<KER, float><<<c.nE, 256, l8, st>>>(nc);                                                \
@@ -3094,7 +3129,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
             k_cut8<KER, double><<<c.nE, 256, l8, st>>>(nc);                                               \
         }                                                                                                 \
     } while (0)
-        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG) {
+        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG && (size_t)nc.HN * 40 <= 160 * 1024) {
             switch (c.kernel) {
                 case KER_CONSTANT: LAUNCH_CUT8(KER_CONSTANT); break;
                 case KER_PARABOLA: LAUNCH_CUT8(KER_PARABOLA); break;
