@@ -638,15 +638,27 @@ __global__ void k_tcount(int nE, int cap, const uint32_t* nbr, const int* cntF, 
         atomicAdd(&tcnt[e & IDX_MASK], 1);
     }
 }
+// Row b of the transposed relation lists the outer elements aT of the pairs (aT, b): pairs
+// from the front segments (interior, close) from the row's start, cut pairs from its end.
+// The cut entries also record the pair's index in the cut-pair enumeration (coff[aT] + its
+// position in aT's back segment); tfillB[b] ends as the number of cut entries of row b.
 __global__ void k_tfill(int nE, int cap, const uint32_t* nbr, const int* cntF, const int* cntB,
-                        const long long* tptr, int* tfill, int* tadj) {
+                        const long long* tptr, int* tfill, int* tadj, const long long* coff, int* tfillB,
+                        int* tcut) {
     int aT = blockIdx.x;
-    int n = cntF[aT] + cntB[aT];
+    const int nF = cntF[aT];
+    int n = nF + cntB[aT];
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        uint32_t e = nbr[(size_t)aT * cap + list_index(p, cntF[aT], cap)];
+        uint32_t e = nbr[(size_t)aT * cap + list_index(p, nF, cap)];
         int b = (int)(e & IDX_MASK);
-        int s = atomicAdd(&tfill[b], 1);
-        tadj[tptr[b] + s] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
+        long long q;
+        if (p < nF) {
+            q = tptr[b] + atomicAdd(&tfill[b], 1);
+        } else {
+            q = tptr[b + 1] - 1 - atomicAdd(&tfillB[b], 1);
+            tcut[q] = (int)(coff[aT] + (p - nF));
+        }
+        tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
     }
 }
 
@@ -905,6 +917,10 @@ struct NumArgs {
     const long long* poff;
     double* P;
     SurArgs sur;
+    // cut pairs: termLocalPrime staged per pair (6 symmetric values at (coff[aT] + p) * 6) for
+    // k_lp_gather; null: added to the matrix directly
+    const long long* coff;
+    double* stage;
 };
 
 // Kernel value(s) at z = x - y: OO entries (row-major od x od).
@@ -1978,6 +1994,7 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     if (kin) for (int i = 0; i < 3; i++) psx[i] = a.q.psix[3 * k + i];
     double sumL = 0;  // innerLocal of this lane's outer point over its pairs
     unsigned mytri = 0;
+    double* const stage = a.sur.list_in ? nullptr : a.stage;
     for (int c0 = 0; c0 < nB; c0 += 32) {
         const int p = c0 + g;
         if (p >= nB) break;  // whole groups leave together
@@ -2020,23 +2037,22 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
             }
         }
         sumL += inL;
-        // this outer point's share of termNonloc (9) and termLocalPrime (9), summed over the group
-        double tN[9], tLP[9];
+        // this outer point's share of termNonloc (9) and termLocalPrime (6, symmetric), summed
+        // over the group
+        double tN[9], tLP[6];
         {
 #pragma clang fp contract(fast)
             for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
-                    tN[3 * i + j] = ak * psx[i] * inN[j];
-                    tLP[3 * i + j] = wk * sLP[sym6(i, j)];
-                }
+                for (int j = 0; j < 3; j++) tN[3 * i + j] = ak * psx[i] * inN[j];
+            for (int m = 0; m < 6; m++) tLP[m] = wk * sLP[m];
         }
 #pragma unroll
-        for (int m = 1; m < 8; m <<= 1)
+        for (int m = 1; m < 8; m <<= 1) {
 #pragma unroll
-            for (int e = 0; e < 9; e++) {
-                tN[e] += shfl_xor_d(tN[e], m);
-                tLP[e] += shfl_xor_d(tLP[e], m);
-            }
+            for (int e = 0; e < 9; e++) tN[e] += shfl_xor_d(tN[e], m);
+#pragma unroll
+            for (int e = 0; e < 6; e++) tLP[e] += shfl_xor_d(tLP[e], m);
+        }
         // -termNonloc (i, j) goes to the entries (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]):
         // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
         // the matrix once per block below
@@ -2047,11 +2063,24 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
                 for (int i = 0; i < 3; i++)
                     atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
         }
-        // termLocalPrime: lane k takes entry (i, j) = k (and lane 0 also entry 8)
-        for (int e = k; e < 9; e += 8) {
-            const int i = e / 3, j = e % 3;
-            if (row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG))
-                atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], tLP[e]);  // (bAdx[j], bAdx[i])
+        // termLocalPrime: staged for k_lp_gather, or lane k adds entry (i, j) = k (and lane 0
+        // also entry 8) to (bAdx[j], bAdx[i])
+        if (stage) {
+            if (k < 6) {
+                double v = tLP[0];
+#pragma unroll
+                for (int m = 1; m < 6; m++) v = k == m ? tLP[m] : v;
+                stage[(a.coff[aT] + p) * 6 + k] = v;
+            }
+        } else {
+            for (int e = k; e < 9; e += 8) {
+                const int i = e / 3, j = e % 3;
+                double v = tLP[0];
+#pragma unroll
+                for (int m = 1; m < 6; m++) v = sym6(i, j) == m ? tLP[m] : v;
+                if (row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG))
+                    atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], v);
+            }
         }
     }
     // termLocal[i][j] = sum_k |aT| dx_k psi_i(x_k) psi_j(x_k) innerLocal_k: sum over the lanes of
@@ -2084,6 +2113,36 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
             }
             atomicAdd(&a.val[a.posTT[9 * aT + 3 * i + j]], v);
         }
+    }
+}
+
+// termLocalPrime of the cut pairs staged by k_cut8, summed per inner element: one wavefront
+// per element b over the cut entries at the end of its transposed row, then the nine entries
+// (bAdx[j], bAdx[i]) of rows that are assembled (src/Cassemble.cpp:756-767).
+__global__ __launch_bounds__(256) void k_lp_gather(int nE, int is_DG, const int* tri, const int* vlabel,
+                                                   const long long* tptr, const int* tcB, const int* tcut,
+                                                   const double* stage, const int* posTT, double* val) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nE) return;
+    const long long e1 = tptr[b + 1], e0 = e1 - tcB[b];
+    if (e0 == e1) return;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (long long q = e0 + lane; q < e1; q += 64) {
+        const double2* st = reinterpret_cast<const double2*>(stage + (size_t)tcut[q] * 6);
+        const double2 v0 = st[0], v1 = st[1], v2 = st[2];
+        s[0] += v0.x; s[1] += v0.y; s[2] += v1.x; s[3] += v1.y; s[4] += v2.x; s[5] += v2.y;
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+#pragma unroll
+        for (int e = 0; e < 6; e++) s[e] += shfl_xor_d(s[e], m);
+    if (lane < 9) {
+        const int i = lane / 3, j = lane % 3;
+        double v = s[0];
+#pragma unroll
+        for (int m = 1; m < 6; m++) v = sym6(i, j) == m ? s[m] : v;
+        if (row_labelled(vlabel, node_of(tri, b, j, is_DG), is_DG)) atomicAdd(&val[posTT[9 * b + 3 * j + i]], v);
     }
 }
 
@@ -2878,8 +2937,14 @@ void plan_assemble(Plan& P, const Estimates& est) {
     HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     dalloc(P.d_tadj, (size_t)ntadj);
+    dalloc(P.d_tcut, (size_t)ntadj);
+    dalloc(P.d_coff, (size_t)c.nE + 1);
+    dalloc(P.d_tcB, (size_t)c.nE);
+    scan_exclusive<int>(P.d_cntB, P.d_coff, c.nE, scan_scratch(P), st);
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
-    k_tfill<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, d_cnt, P.d_tadj);
+    HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
+    k_tfill<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, d_cnt, P.d_tadj, P.d_coff,
+                                  P.d_tcB, P.d_tcut);
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);
 
@@ -3130,11 +3195,30 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                                                 \
     } while (0)
         if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG && (size_t)nc.HN * 40 <= 160 * 1024) {
+            if (!use_sur) {
+                // termLocalPrime staged per cut pair, summed per inner element by k_lp_gather
+                long long ncut = 0;
+                HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipStreamSynchronize(st));
+                if (ncut <= INT_MAX) {  // the transposed cut entries hold int32 pair indices
+                    if (P.stage_cap < ncut || !P.d_stage) {
+                        dalloc(P.d_stage, (size_t)std::max<long long>(ncut, 1) * 6);
+                        P.stage_cap = std::max<long long>(ncut, 1);
+                    }
+                    nc.coff = P.d_coff;
+                    nc.stage = P.d_stage;
+                }
+            }
             switch (c.kernel) {
                 case KER_CONSTANT: LAUNCH_CUT8(KER_CONSTANT); break;
                 case KER_PARABOLA: LAUNCH_CUT8(KER_PARABOLA); break;
                 case KER_FRACTIONAL: LAUNCH_CUT8(KER_FRACTIONAL); break;
                 default: LAUNCH_CUT8(KER_MICRO); break;
+            }
+            if (nc.stage) {
+                HIPCHECK(hipGetLastError());
+                k_lp_gather<<<(c.nE + 3) / 4, 256, 0, st>>>(c.nE, c.is_DG, P.d_tri, P.d_vlabel, P.d_tptr, P.d_tcB,
+                                                             P.d_tcut, P.d_stage, P.d_posTT, P.d_val);
             }
         } else {
             switch (c.kernel) {
@@ -3275,7 +3359,7 @@ void plan_free(Plan& P) {
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
-                    P.d_poff, P.d_P, P.d_adjp};
+                    P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < 2 * PH_COUNT; i++)

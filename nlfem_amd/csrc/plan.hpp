@@ -97,6 +97,11 @@ struct Plan {
     // transpose of the neighbour relation
     long long* d_tptr = nullptr;
     int* d_tadj = nullptr;
+    long long* d_coff = nullptr;  // [nE+1] exclusive scan of the cut-pair counts
+    int* d_tcB = nullptr;         // [nE] cut entries at the end of each transposed row
+    int* d_tcut = nullptr;        // [ntadj] cut entries: index of the pair in the cut enumeration
+    double* d_stage = nullptr;    // [ncut*6] termLocalPrime per cut pair
+    long long stage_cap = 0;
     // node CSR (pattern of the full symmetric matrix in node space, od x od blocks per entry)
     long long* d_rowptr = nullptr;  // [nNodes+1]
     int* d_col = nullptr;           // [nnz_nodes]
