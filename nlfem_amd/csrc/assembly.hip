@@ -630,35 +630,81 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
 
 // ======================================================================= transpose
-__global__ void k_tcount(int nE, int cap, const uint32_t* nbr, const int* cntF, const int* cntB, int* tcnt) {
-    int aT = blockIdx.x;
-    int n = cntF[aT] + cntB[aT];
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        uint32_t e = nbr[(size_t)aT * cap + list_index(p, cntF[aT], cap)];
-        atomicAdd(&tcnt[e & IDX_MASK], 1);
+// Transposed relation.  Row b lists the outer elements aT of the pairs (aT, b): pairs from
+// the front segments (interior, close) from the row's start, cut pairs from its end.  The cut
+// entries also record the pair's index in the cut-pair enumeration (coff[aT] + its position
+// in aT's back segment).  A block takes tg consecutive outer elements, whose lists share most
+// of their inner elements: per-b counts are summed in an LDS hash first, so each block issues
+// one global atomic per distinct b (with a bounded probe; keys that find no slot use global
+// atomics per entry).
+constexpr int TH = 4096;
+NLFEM_DEV int lds_slot_b(int* h, int key) {  // insert with at most 64 probes; -1 if none free
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(TH - 1);
+    for (int p = 0; p < 64; p++) {
+        int old = atomicCAS(&h[s], -1, key);
+        if (old == -1 || old == key) return (int)s;
+        s = (s + 1) & (uint32_t)(TH - 1);
     }
+    return -1;
 }
-// Row b of the transposed relation lists the outer elements aT of the pairs (aT, b): pairs
-// from the front segments (interior, close) from the row's start, cut pairs from its end.
-// The cut entries also record the pair's index in the cut-pair enumeration (coff[aT] + its
-// position in aT's back segment); tfillB[b] ends as the number of cut entries of row b.
-__global__ void k_tfill(int nE, int cap, const uint32_t* nbr, const int* cntF, const int* cntB,
-                        const long long* tptr, int* tfill, int* tadj, const long long* coff, int* tfillB,
-                        int* tcut) {
-    int aT = blockIdx.x;
-    const int nF = cntF[aT];
-    int n = nF + cntB[aT];
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        uint32_t e = nbr[(size_t)aT * cap + list_index(p, nF, cap)];
-        int b = (int)(e & IDX_MASK);
-        long long q;
-        if (p < nF) {
-            q = tptr[b] + atomicAdd(&tfill[b], 1);
-        } else {
-            q = tptr[b + 1] - 1 - atomicAdd(&tfillB[b], 1);
-            tcut[q] = (int)(coff[aT] + (p - nF));
+NLFEM_DEV int lds_find_b(const int* h, int key) {
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(TH - 1);
+    for (int p = 0; p < 64; p++) {
+        int k = h[s];
+        if (k == key) return (int)s;
+        if (k == -1) return -1;
+        s = (s + 1) & (uint32_t)(TH - 1);
+    }
+    return -1;
+}
+
+// FILL = false: tcnt[b] += entries of row b.  FILL = true: place the entries; tfill / tfillB
+// count the front / back entries placed so far (tfillB[b] ends as row b's cut-entry count).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
+                                                   const int* cntB, int* tcnt, const long long* tptr, int* tfill,
+                                                   int* tadj, const long long* coff, int* tfillB, int* tcut) {
+    __shared__ int hk[TH], hf[TH], hb[TH];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < TH; i += 256) { hk[i] = -1; hf[i] = 0; hb[i] = 0; }
+    __syncthreads();
+    const int a0 = blockIdx.x * tg, a1 = min(a0 + tg, nE);
+    for (int aT = a0; aT < a1; aT++) {
+        const int nF = cntF[aT], n = nF + cntB[aT];
+        for (int p = tid; p < n; p += 256) {
+            const int b = (int)(nbr[(size_t)aT * cap + list_index(p, nF, cap)] & IDX_MASK);
+            const int sl = lds_slot_b(hk, b);
+            if (sl >= 0) atomicAdd(p < nF || !FILL ? &hf[sl] : &hb[sl], 1);
+            else if (!FILL) atomicAdd(&tcnt[b], 1);
         }
-        tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
+    }
+    __syncthreads();
+    for (int i = tid; i < TH; i += 256) {
+        const int b = hk[i];
+        if (b < 0) continue;
+        if (!FILL) atomicAdd(&tcnt[b], hf[i]);
+        else {  // reserve the block's ranges; hf / hb become the next free rank
+            if (hf[i]) hf[i] = atomicAdd(&tfill[b], hf[i]);
+            if (hb[i]) hb[i] = atomicAdd(&tfillB[b], hb[i]);
+        }
+    }
+    if (!FILL) return;
+    __syncthreads();
+    for (int aT = a0; aT < a1; aT++) {
+        const int nF = cntF[aT], n = nF + cntB[aT];
+        for (int p = tid; p < n; p += 256) {
+            const uint32_t e = nbr[(size_t)aT * cap + list_index(p, nF, cap)];
+            const int b = (int)(e & IDX_MASK);
+            const int sl = lds_find_b(hk, b);
+            long long q;
+            if (p < nF) {
+                q = tptr[b] + (sl >= 0 ? atomicAdd(&hf[sl], 1) : atomicAdd(&tfill[b], 1));
+            } else {
+                q = tptr[b + 1] - 1 - (sl >= 0 ? atomicAdd(&hb[sl], 1) : atomicAdd(&tfillB[b], 1));
+                tcut[q] = (int)(coff[aT] + (p - nF));
+            }
+            tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
+        }
     }
 }
 
@@ -2576,6 +2622,18 @@ static void dalloc(Tp*& p, size_t n) {
     HIPCHECK(hipMalloc((void**)&p, n * sizeof(Tp)));
 }
 
+// Plan-owned buffer of at least n elements: kept when it is large enough (its contents are
+// not preserved otherwise).
+template <typename Tp>
+static void dreserve(Plan& P, Tp*& p, size_t n) {
+    if (n == 0) n = 1;
+    size_t& cap = P.dcap[(const void*)&p];
+    if (p && cap >= n) return;
+    if (p) { (void)hipFree(p); p = nullptr; }
+    HIPCHECK(hipMalloc((void**)&p, n * sizeof(Tp)));
+    cap = n;
+}
+
 static void read_counters(Plan& P, unsigned long long* h) {
     HIPCHECK(hipMemcpyAsync(h, P.d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, P.stream));
     HIPCHECK(hipStreamSynchronize(P.stream));
@@ -2798,24 +2856,24 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- dual graph
     ev_begin(P, PH_DUAL);
-    int* d_cnt = nullptr;
-    dalloc(d_cnt, (size_t)std::max(c.nE, c.nV) + 1);
+    int*& d_cnt = P.d_wcnt;
+    dreserve(P, d_cnt, (size_t)std::max(c.nE, c.nV) + 1);
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nV, st));
-    dalloc(P.d_vptr, (size_t)c.nV + 1);
-    dalloc(P.d_vel, 3 * (size_t)c.nE);
+    dreserve(P, P.d_vptr, (size_t)c.nV + 1);
+    dreserve(P, P.d_vel, 3 * (size_t)c.nE);
     k_vinc_count<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, d_cnt);
     scan_exclusive<int>(d_cnt, P.d_vptr, c.nV, scan_scratch(P), st);
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nV, st));
     k_vinc_fill<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, P.d_vptr, d_cnt, P.d_vel);
     k_sort_vinc<<<nblk(c.nV, 256), 256, 0, st>>>(c.nV, P.d_vptr, P.d_vel);
-    dalloc(P.d_xadj, (size_t)c.nE + 1);
+    dreserve(P, P.d_xadj, (size_t)c.nE + 1);
     k_dual<false><<<nblk(c.nE, 128), 128, 0, st>>>(c.nE, P.d_tri, P.d_vptr, P.d_vel, d_cnt, nullptr, nullptr,
                                                    P.d_counters);
     scan_exclusive<int>(d_cnt, P.d_xadj, c.nE, scan_scratch(P), st);
     long long nadj = 0;
     HIPCHECK(hipMemcpyAsync(&nadj, P.d_xadj + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    dalloc(P.d_adj, (size_t)nadj);
+    dreserve(P, P.d_adj, (size_t)nadj);
     k_dual<true><<<nblk(c.nE, 128), 128, 0, st>>>(c.nE, P.d_tri, P.d_vptr, P.d_vel, nullptr, P.d_xadj, P.d_adj,
                                                   P.d_counters);
     HIPCHECK(hipGetLastError());
@@ -2823,7 +2881,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     const int maxdeg = (int)h[CNT_MAXT];
     int W4 = (maxdeg + 3) / 4;
     if (W4 > BFS_W4MAX) W4 = 0;  // irregular meshes: BFS walks the CSR rows
-    if (W4 > 0) dalloc(P.d_adjp, (size_t)c.nE * 4 * W4);
+    if (W4 > 0) dreserve(P, P.d_adjp, (size_t)c.nE * 4 * W4);
     HIPCHECK(hipMemsetAsync(P.d_counters + CNT_MAXROW, 0, sizeof(unsigned long long), st));
     k_adjpad<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_xadj, P.d_adj, 4 * W4, W4 > 0 ? P.d_adjp : nullptr,
                                                P.d_counters + CNT_MAXROW);
@@ -2837,10 +2895,10 @@ void plan_assemble(Plan& P, const Estimates& est) {
     double area_est = M_PI * std::pow(delta + 2.0 * est.hmax, 2) / est.aref();
     long long visit = c.fullCC ? (long long)c.nE : (long long)(1.3 * area_est) + 128;
     if (visit > c.nE) visit = c.nE;
-    dalloc(P.d_cntF, (size_t)c.nE);
-    dalloc(P.d_cntB, (size_t)c.nE);
+    dreserve(P, P.d_cntF, (size_t)c.nE);
+    dreserve(P, P.d_cntB, (size_t)c.nE);
     const long long closeCap = (long long)c.nE * 32 + 1;
-    dalloc(P.d_closeList, (size_t)closeCap);
+    dreserve(P, P.d_closeList, (size_t)closeCap);
     BfsArgs ba;
     ba.nE = c.nE; ba.elem = P.d_elem; ba.xadj = P.d_xadj; ba.adj = P.d_adj; ba.q = c.q;
     ba.adjp = P.d_adjp; ba.W4 = W4;
@@ -2884,7 +2942,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }
         ba.whalf = (int)whalf;
         P.cap = (int)cap;
-        dalloc(P.d_nbr, (size_t)c.nE * (size_t)cap);
+        dreserve(P, P.d_nbr, (size_t)c.nE * (size_t)cap);
         ba.cap = (int)cap; ba.H = (int)H; ba.F = (int)F;
         ba.nbr = P.d_nbr; ba.cntF = P.d_cntF; ba.cntB = P.d_cntB; ba.closeList = P.d_closeList;
         ba.counters = P.d_counters;
@@ -2929,22 +2987,28 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     // ---------------------------------------------------------------- transpose
     ev_begin(P, PH_TRANSPOSE);
+    // outer elements per transpose block: with a local element numbering (the BFS stayed in its
+    // index window) consecutive elements share most of their lists, and eight lists of up to
+    // 1400 entries keep the LDS hash under half full
+    const int tg = (use_bm && P.stats.max_list <= 1400) ? 8 : 1;
+    const int tblocks = (c.nE + tg - 1) / tg;
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
-    k_tcount<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt);
-    dalloc(P.d_tptr, (size_t)c.nE + 1);
+    k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
+                                                nullptr, nullptr, nullptr, nullptr, nullptr);
+    dreserve(P, P.d_tptr, (size_t)c.nE + 1);
     scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
     long long ntadj = 0;
     HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    dalloc(P.d_tadj, (size_t)ntadj);
-    dalloc(P.d_tcut, (size_t)ntadj);
-    dalloc(P.d_coff, (size_t)c.nE + 1);
-    dalloc(P.d_tcB, (size_t)c.nE);
+    dreserve(P, P.d_tadj, (size_t)ntadj);
+    dreserve(P, P.d_tcut, (size_t)ntadj);
+    dreserve(P, P.d_coff, (size_t)c.nE + 1);
+    dreserve(P, P.d_tcB, (size_t)c.nE);
     scan_exclusive<int>(P.d_cntB, P.d_coff, c.nE, scan_scratch(P), st);
     HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
     HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
-    k_tfill<<<c.nE, 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, d_cnt, P.d_tadj, P.d_coff,
-                                  P.d_tcB, P.d_tcut);
+    k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
+                                               d_cnt, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);
 
@@ -2956,10 +3020,10 @@ void plan_assemble(Plan& P, const Estimates& est) {
     pa.nNodes = P.nNodes; pa.is_DG = c.is_DG; pa.cap = P.cap; pa.tri = P.d_tri; pa.vptr = P.d_vptr;
     pa.vel = P.d_vel; pa.nbr = P.d_nbr; pa.cntF = P.d_cntF; pa.cntB = P.d_cntB; pa.tptr = P.d_tptr;
     pa.tadj = P.d_tadj; pa.counters = P.d_counters;
-    int* d_rowcnt = nullptr;
-    dalloc(d_rowcnt, (size_t)P.nNodes + 1);
-    dalloc(P.d_rowptr, (size_t)P.nNodes + 1);
-    int* d_ptmp = nullptr;
+    int*& d_rowcnt = P.d_wrowcnt;
+    dreserve(P, d_rowcnt, (size_t)P.nNodes + 1);
+    dreserve(P, P.d_rowptr, (size_t)P.nNodes + 1);
+    int*& d_ptmp = P.d_wptmp;
     long long rc = next_pow2((long long)(0.6 * (c.is_DG ? 4 * ecap : ecap)) + 64);
     // element set as a bitmap: the row's elements lie within levels + 2 dual-graph steps of
     // the row's first element, each step moving at most `bandwidth` indices
@@ -2974,7 +3038,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         if (rc > pa.DH) rc = pa.DH;
         pa.SB = 0;
         pa.RC = (int)rc;
-        dalloc(d_ptmp, (size_t)P.nNodes * (size_t)rc);
+        dreserve(P, d_ptmp, (size_t)P.nNodes * (size_t)rc);
         pa.tmp = d_ptmp;
         pa.rowcnt = d_rowcnt;
         pa.rowptr = nullptr;
@@ -3006,7 +3070,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     HIPCHECK(hipMemcpyAsync(&P.nnz_nodes, P.d_rowptr + P.nNodes, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     if (P.nnz_nodes >= (1ll << 31)) throw std::length_error("pattern exceeds 2^31 node entries");
-    dalloc(P.d_col, (size_t)P.nnz_nodes);
+    dreserve(P, P.d_col, (size_t)P.nnz_nodes);
     pa.SB = (int)next_pow2(std::max(P.maxRow, 1));
     pa.rowptr = P.d_rowptr;
     pa.col = P.d_col;
@@ -3017,23 +3081,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
         k_pattern_sort<<<P.nNodes, 256, ldsb, st>>>(pa);
         HIPCHECK(hipGetLastError());
     }
-    (void)hipFree(d_ptmp);
     ev_end(P, PH_PATTERN);
 
     // ---------------------------------------------------------------- positions
     ev_begin(P, PH_POSITIONS);
-    int* d_rowOf = nullptr;
-    dalloc(d_rowOf, (size_t)P.nnz_nodes);
-    dalloc(P.d_tpos, (size_t)P.nnz_nodes);
-    dalloc(P.d_posTT, 9 * (size_t)c.nE);
+    int*& d_rowOf = P.d_wrowOf;
+    dreserve(P, d_rowOf, (size_t)P.nnz_nodes);
+    dreserve(P, P.d_tpos, (size_t)P.nnz_nodes);
+    dreserve(P, P.d_posTT, 9 * (size_t)c.nE);
     k_row_of<<<P.nNodes, 256, 0, st>>>(P.nNodes, P.d_rowptr, d_rowOf);
     k_tpos<<<nblk(P.nnz_nodes, 256), 256, 0, st>>>(P.nnz_nodes, P.d_rowptr, P.d_col, d_rowOf, P.d_tpos, P.d_counters);
     k_posTT<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_tri, c.is_DG, P.d_rowptr, P.d_col, P.d_posTT, P.d_counters);
-    dalloc(P.d_val, (size_t)P.nnz_nodes * P.oo);
+    dreserve(P, P.d_val, (size_t)P.nnz_nodes * P.oo);
     HIPCHECK(hipMemsetAsync(P.d_val, 0, sizeof(double) * (size_t)P.nnz_nodes * P.oo, st));
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_POSITIONS);
-    (void)hipFree(d_rowOf);
 
     // ---------------------------------------------------------------- quadrature
     NumArgs na = base_numargs(P);
@@ -3045,17 +3107,16 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // interior pairs: owner computes into partial rows, then merge (plain stores into val)
     ev_begin(P, PH_INTERIOR);
     {
-        int* d_plen = nullptr;
-        dalloc(d_plen, 3 * (size_t)c.nE);
-        dalloc(P.d_poff, 3 * (size_t)c.nE + 1);
+        int*& d_plen = P.d_wplen;
+        dreserve(P, d_plen, 3 * (size_t)c.nE);
+        dreserve(P, P.d_poff, 3 * (size_t)c.nE + 1);
         k_plen<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.d_elem, P.d_tri, P.d_vlabel, c.is_DG, P.d_rowptr, d_plen);
         scan_exclusive<int>(d_plen, P.d_poff, 3LL * c.nE, scan_scratch(P), st);
         long long ptotal = 0;
         HIPCHECK(hipMemcpyAsync(&ptotal, P.d_poff + 3LL * c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        (void)hipFree(d_plen);
         if (P.ptotal < ptotal * P.oo) {
-            dalloc(P.d_P, (size_t)ptotal * P.oo);
+            dreserve(P, P.d_P, (size_t)ptotal * P.oo);
             P.ptotal = ptotal * P.oo;
         }
         na.tptr = P.d_tptr; na.tadj = P.d_tadj; na.poff = P.d_poff; na.P = P.d_P;
@@ -3144,14 +3205,14 @@ void plan_assemble(Plan& P, const Estimates& est) {
         NumArgs nc = na;
         if (use_sur) {
             // surrogate first; the pairs it hands back go through the exact kernel below
-            dalloc(P.d_roff, (size_t)c.nE + 1);
-            dalloc(P.d_rcnt, (size_t)c.nE);
+            dreserve(P, P.d_roff, (size_t)c.nE + 1);
+            dreserve(P, P.d_rcnt, (size_t)c.nE);
             scan_exclusive<int>(P.d_cntB, P.d_roff, c.nE, scan_scratch(P), st);
             long long ncut = 0;
             HIPCHECK(hipMemcpyAsync(&ncut, P.d_roff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
             if (P.rlist_cap < ncut || !P.d_rlist) {
-                dalloc(P.d_rlist, (size_t)std::max<long long>(ncut, 1));
+                dreserve(P, P.d_rlist, (size_t)std::max<long long>(ncut, 1));
                 P.rlist_cap = std::max<long long>(ncut, 1);
             }
             NumArgs ns = na;
@@ -3202,7 +3263,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
                 HIPCHECK(hipStreamSynchronize(st));
                 if (ncut <= INT_MAX) {  // the transposed cut entries hold int32 pair indices
                     if (P.stage_cap < ncut || !P.d_stage) {
-                        dalloc(P.d_stage, (size_t)std::max<long long>(ncut, 1) * 6);
+                        dreserve(P, P.d_stage, (size_t)std::max<long long>(ncut, 1) * 6);
                         P.stage_cap = std::max<long long>(ncut, 1);
                     }
                     nc.coff = P.d_coff;
@@ -3240,24 +3301,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // ---------------------------------------------------------------- compaction
     ev_begin(P, PH_COMPACT);
     const long long K = c.K;
-    int* d_ocnt = nullptr;
-    dalloc(d_ocnt, (size_t)K + 1);
+    int*& d_ocnt = P.d_wocnt;
+    dreserve(P, d_ocnt, (size_t)K + 1);
     k_compact<false><<<P.nNodes, 64, 0, st>>>(P.nNodes, c.outdim, P.d_rowptr, P.d_col, P.d_val, d_ocnt, nullptr,
                                               nullptr, nullptr);
-    dalloc(P.d_orow, (size_t)K + 1);
+    dreserve(P, P.d_orow, (size_t)K + 1);
     scan_exclusive<int>(d_ocnt, P.d_orow, K, scan_scratch(P), st);
     HIPCHECK(hipMemcpyAsync(&P.nnz, P.d_orow + K, sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    dalloc(P.d_ocol, (size_t)P.nnz);
-    dalloc(P.d_oval, (size_t)P.nnz);
+    dreserve(P, P.d_ocol, (size_t)P.nnz);
+    dreserve(P, P.d_oval, (size_t)P.nnz);
     k_compact<true><<<P.nNodes, 64, 0, st>>>(P.nNodes, c.outdim, P.d_rowptr, P.d_col, P.d_val, nullptr, P.d_orow,
                                              P.d_ocol, P.d_oval);
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_COMPACT);
     read_counters(P, h);
-    (void)hipFree(d_ocnt);
-    (void)hipFree(d_cnt);
-    (void)hipFree(d_rowcnt);
     if (h[CNT_ERROR]) throw std::runtime_error("assembly consistency error (code " + std::to_string(h[CNT_ERROR]) + ")");
     P.stats.nnz_nodes = P.nnz_nodes;
     P.stats.nnz = P.nnz;
@@ -3361,6 +3419,9 @@ void plan_free(Plan& P) {
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
                     P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage};
     for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    void* wp[] = {P.d_wcnt, P.d_wrowcnt, P.d_wptmp, P.d_wrowOf, P.d_wplen, P.d_wocnt};
+    for (void* p : wp)
         if (p) (void)hipFree(p);
     for (int i = 0; i < 2 * PH_COUNT; i++)
         if (P.ev[i]) (void)hipEventDestroy(P.ev[i]);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "nlfem_device.hpp"
@@ -124,6 +125,15 @@ struct Plan {
     // scratch
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+    int* d_wcnt = nullptr;     // per-element / per-vertex counters
+    int* d_wrowcnt = nullptr;  // pattern row lengths
+    int* d_wptmp = nullptr;    // unsorted pattern rows
+    int* d_wrowOf = nullptr;   // row of each node-CSR entry
+    int* d_wplen = nullptr;    // partial-row lengths
+    int* d_wocnt = nullptr;    // output row lengths
+    // allocated capacity (elements) of each device buffer above, keyed by the member's address:
+    // repeated assemblies reuse buffers that are large enough
+    std::unordered_map<const void*, size_t> dcap;
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
