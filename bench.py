@@ -13,6 +13,7 @@ same workload, on this host's cores).
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -51,6 +52,23 @@ def conf_uses_cut8(conf, kernel):
     """The library's cut-pair kernel for scalar kernels with <= 8 outer points and CG."""
     return (kernel.get("outputdim", 1) == 1 and conf["quadrature"]["outer"]["points"].shape[0] <= 8
             and conf.get("ansatz", "CG") == "CG")
+
+
+def measured_traffic(kernel_name, workload):
+    """HBM bytes per launch of `kernel_name` on `workload`, from the committed counter summary
+    profiles/traffic__<kernel>_<workload>_fp64.md (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    of this bench, each scaled by what the counter reads for a plain copy of known size);
+    None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic__{kernel_name}_{workload}_fp64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"\s*-?\s*HBM bytes per launch:\s*([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(mesh, kernel, conf, budget_s=15.0):
@@ -135,6 +153,8 @@ def main():
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": None, "flops_per_launch": f_dom, "ms_per_launch": round(t_dom * 1000, 4)}
+    if args.precision == "fp64":
+        roof["traffic"] = measured_traffic(roof["kernel"], args.workload)
     out = {
         "metric": "element-pairs/s (assembled ordered element pairs per second, full assembly)",
         "value": value, "unit": "element-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -91,8 +91,10 @@ int nlfem_plan_csr_device(const nlfem_plan* plan, long long** rowptr, int** col,
 int nlfem_plan_neighbours_total(const nlfem_plan* plan, long long* total);
 int nlfem_plan_neighbours(const nlfem_plan* plan, long long* ptr, int* nbr, unsigned char* cls);
 /* Statistics: out[0..] = pairs, probes, close, interior, cut, bary, nnz_nodes, nnz, cap,
- * max_list, subtri (fan triangles integrated by the cut kernel), then per-phase milliseconds (dual, bfs, transpose, pattern, positions, close,
- * interior, cut, compact).  n = number of doubles available in out; returns NLFEM_OK. */
+ * max_list, subtri (fan triangles integrated by the cut kernel), then per-phase milliseconds
+ * (dual, bfs, transpose, pattern, positions, close, interior, cut, compact, gather: the
+ * per-inner-element sum of the cut pairs' staged termLocalPrime, which runs between cut and
+ * compact).  n = number of doubles available in out; returns NLFEM_OK. */
 int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
 /* Cut pairs through the surrogate network (no counterpart in the reference; constant kernel,
  * retriangulate).  nnet = 1 or 2 networks (params[nnet][9737], norm[nnet][42], see the

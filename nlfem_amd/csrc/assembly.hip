@@ -1391,6 +1391,94 @@ __global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
 // around each node.  No global atomics.
 constexpr unsigned long long ERR_HASH = 128ull;
 
+// Exclusive prefix sum over a 256-thread block (tmp: 4 ints of LDS); total returned in *tot.
+NLFEM_DEV int block_exscan256(int v, int* tmp, int* tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int off = 0, all = 0;
+    for (int i = 0; i < 4; i++) {
+        if (i < w) off += tmp[i];
+        all += tmp[i];
+    }
+    __syncthreads();
+    *tot = all;
+    return off + x - v;
+}
+
+// Column map of an outer element's node rows: node id -> index into the block's column
+// accumulators (< H).  Rows are sorted, so their nodes lie in [first, last] of the rows.
+// Rank mode, when that window spans at most 16 H nodes: a bitmap of the window (WW words)
+// and per-word prefix counts (WW ints) in the same H ints of LDS; a node's index is its rank
+// among the rows' nodes, found with two LDS reads and a popcount.  Otherwise an LDS hash of
+// H slots (index = slot, found by probing).  Build with colmap_build by all 256 threads.
+struct ColMap {
+    int* h;
+    int H, base, WW;  // WW > 0: rank mode
+    NLFEM_DEV int find(int n) const {
+        if (WW > 0) {
+            const unsigned d = (unsigned)(n - base);
+            if (d >= 32u * (unsigned)WW) return -1;
+            const unsigned w = (unsigned)h[d >> 5], m = 1u << (d & 31);
+            const int r = (w & m) ? h[WW + (d >> 5)] + __popc(w & (m - 1)) : -1;
+            return r < H ? r : -1;  // more nodes than accumulators: reported by colmap_build
+        }
+        return lds_find(h, H, n);
+    }
+};
+// rows: use[j] selects which of the three rows (rp[j], rlen[j] entries of col) are mapped.
+NLFEM_DEV ColMap colmap_build(int* h, int H, const int* col, const long long* rp, const int* rlen, const bool* use,
+                              int* tmp, unsigned long long* counters) {
+    ColMap cm;
+    cm.h = h;
+    cm.H = H;
+    int lo = INT_MAX, hi = -1;
+    for (int j = 0; j < 3; j++)
+        if (use[j] && rlen[j] > 0) {
+            lo = min(lo, col[rp[j]]);
+            hi = max(hi, col[rp[j] + rlen[j] - 1]);
+        }
+    const long long span = (long long)hi - lo + 1;
+    cm.base = lo;
+    cm.WW = (hi >= lo && span <= 16LL * H) ? (int)((span + 31) / 32) : 0;
+    const int tid = threadIdx.x;
+    if (cm.WW > 0) {
+        for (int i = tid; i < cm.WW; i += 256) h[i] = 0;
+        __syncthreads();
+        for (int j = 0; j < 3; j++)
+            if (use[j])
+                for (int i = tid; i < rlen[j]; i += 256) {
+                    const unsigned d = (unsigned)(col[rp[j] + i] - lo);
+                    atomicOr(&h[d >> 5], (int)(1u << (d & 31)));
+                }
+        __syncthreads();
+        const int per = (cm.WW + 255) / 256, w0 = min(tid * per, cm.WW), w1 = min(w0 + per, cm.WW);
+        int c = 0;
+        for (int i = w0; i < w1; i++) c += __popc((unsigned)h[i]);
+        int tot;
+        int off = block_exscan256(c, tmp, &tot);
+        for (int i = w0; i < w1; i++) {
+            h[cm.WW + i] = off;
+            off += __popc((unsigned)h[i]);
+        }
+        if (tid == 0 && tot > H) atomicOr(&counters[CNT_ERROR], ERR_HASH);
+    } else {
+        for (int i = tid; i < H; i += 256) h[i] = -1;
+        __syncthreads();
+        for (int j = 0; j < 3; j++)
+            if (use[j])
+                for (int i = tid; i < rlen[j]; i += 256)
+                    if (lds_slot(h, H, col[rp[j] + i]) < 0) atomicOr(&counters[CNT_ERROR], ERR_HASH);
+    }
+    __syncthreads();
+    return cm;
+}
+
 template <int KER, int OD, typename T>
 __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
@@ -1409,17 +1497,15 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
     }
     if (!(labA[0] || labA[1] || labA[2])) return;
-    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    __shared__ int scan_tmp[4];
     for (int i = tid; i < a.HN * 3 * OO; i += 256) acc[i] = 0.0;
-    __syncthreads();
+    long long rp[3];
+    int rlen[3];
     for (int j = 0; j < 3; j++) {
-        if (!labA[j]) continue;
-        const long long r0 = a.rowptr[rA[j]];
-        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
-        for (int i = tid; i < len; i += 256)
-            if (lds_slot(hk, a.HN, a.col[r0 + i]) < 0) atomicOr(&a.counters[CNT_ERROR], ERR_HASH);
+        rp[j] = a.rowptr[rA[j]];
+        rlen[j] = (int)(a.rowptr[rA[j] + 1] - rp[j]);
     }
-    __syncthreads();
+    const ColMap cm = colmap_build(hk, a.HN, a.col, rp, rlen, labA, scan_tmp, a.counters);
     double tL[NB], tLP[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) { tL[i] = 0; tLP[i] = 0; }
@@ -1438,7 +1524,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
             const double w = a.kp.c * A.absDet * a.elem[U].absDet;
             sumB += a.elem[U].absDet;
             for (int j = 0; j < 3; j++) {  // column U_j collects w q1_j; row T_i takes p1_i of it below
-                const int s = lds_find(hk, a.HN, node_of(a.tri, U, j, a.is_DG));
+                const int s = cm.find(node_of(a.tri, U, j, a.is_DG));
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
                 atomicAdd(&acc[s * 3 + 0], w * a.q1[j]);
             }
@@ -1452,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
             const double w = a.kp.c * absU * A.absDet;
             sumU += absU;
             for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i): column U_i collects w p1_i
-                const int s = lds_find(hk, a.HN, node_of(a.tri, U, i, a.is_DG));
+                const int s = cm.find(node_of(a.tri, U, i, a.is_DG));
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
                 atomicAdd(&acc[s * 3 + 1], w * a.p1[i]);
             }
@@ -1472,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         PairBlocks<OD> pb;
         pair_interior<KER, OD, T, true, false>(a, A, B, tL, pb);
         for (int j = 0; j < 3; j++) {
-            const int s = lds_find(hk, a.HN, node_of(a.tri, U, j, a.is_DG));
+            const int s = cm.find(node_of(a.tri, U, j, a.is_DG));
             for (int i = 0; i < 3; i++) {
                 if (!labA[i]) continue;
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
@@ -1496,7 +1582,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
 #pragma unroll
         for (int t = 0; t < NB; t++) tLP[t] += pb.tLP[t];
         for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i)
-            const int s = lds_find(hk, a.HN, node_of(a.tri, U, i, a.is_DG));
+            const int s = cm.find(node_of(a.tri, U, i, a.is_DG));
             for (int j = 0; j < 3; j++) {
                 if (!labA[j]) continue;
                 if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
@@ -1530,11 +1616,11 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
         if (t < NB) {  // +termLocal into (T_i, T_j)
             if (!labA[i]) continue;
-            const int s = lds_find(hk, a.HN, rA[j]);
+            const int s = cm.find(rA[j]);
             if (s >= 0) atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], v);
         } else {       // +termLocalPrime[N a + b] into (T_b, T_a): row T_j comp bo, col T_i comp ao
             if (!labA[j]) continue;
-            const int s = lds_find(hk, a.HN, rA[i]);
+            const int s = cm.find(rA[i]);
             if (s >= 0) atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], v);
         }
     }
@@ -1542,11 +1628,11 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     // partial rows of T's nodes, in column order
     for (int j = 0; j < 3; j++) {
         if (!labA[j]) continue;
-        const long long r0 = a.rowptr[rA[j]];
-        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        const long long r0 = rp[j];
+        const int len = rlen[j];
         double* dst = a.P + a.poff[3 * T0 + j] * OO;
         for (int i = tid; i < len; i += 256) {
-            const int s = lds_find(hk, a.HN, a.col[r0 + i]);
+            const int s = cm.find(a.col[r0 + i]);
 #pragma unroll
             for (int o = 0; o < OO; o++) dst[(long long)i * OO + o] = (s >= 0) ? acc[(s * 3 + j) * OO + o] : 0.0;
         }
@@ -1983,23 +2069,59 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     }
 }
 
+// LDS hash of node ids with at most 64 probes (a full table reports -1 instead of scanning).
+NLFEM_DEV int lds_slot_n(int* h, int H, int key) {
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
+    for (int p = 0; p < 64 && p < H; p++) {
+        int old = atomicCAS(&h[s], -1, key);
+        if (old == -1 || old == key) return (int)s;
+        s = (s + 1) & (uint32_t)(H - 1);
+    }
+    return -1;
+}
+NLFEM_DEV int lds_find_n(const int* h, int H, int key) {
+    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
+    for (int p = 0; p < 64 && p < H; p++) {
+        int k = h[s];
+        if (k == key) return (int)s;
+        if (k == -1) return -1;
+        s = (s + 1) & (uint32_t)(H - 1);
+    }
+    return -1;
+}
+// Offset of `node` in a sorted node-CSR row, or -1.
+NLFEM_DEV int row_search(const int* row, int len, int node) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row[mid] < node) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < len && row[lo] == node) ? lo : -1;
+}
+
 // Cut pairs, scalar kernels with at most 8 outer points: one 256-thread block per outer
 // element; 32 groups of 8 lanes, one group per cut pair and one lane per outer point, so a
 // chunk of 32 pairs needs no block barrier.  Each lane retriangulates B(x_k, delta) n bT and
-// integrates the fan triangles as they are emitted (constant kernel: from the corners and
-// the rule's moments, cut_item_constant; otherwise the inner rule point by point, as
-// pair_cut), forms its outer point's share of termNonloc and termLocalPrime
-// (src/integration.cpp:508-537), and the group sums them with three xor shuffles.  The 9
-// (i, j) entries of the pair are scattered by the group's lanes (src/Cassemble.cpp:721-770).
-// termLocal only reaches the outer element's rows: each lane keeps innerLocal of its outer
-// point summed over all its pairs, and the block forms termLocal once at the end.
+// integrates the fan triangles as they are emitted (constant kernel: ConstantItem's
+// geometric moments; otherwise the inner rule point by point, as the reference's
+// integrate_retriangulate), forms its outer point's share of termNonloc and termLocalPrime
+// (src/integration.cpp:508-537), and the group sums them with three xor shuffles.  Where the
+// blocks go (src/Cassemble.cpp:721-770):
+//   -termNonloc (i, j) to (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]): summed per column node
+//      bAdx[j] in LDS over all pairs of the block (a table of the partners' nodes), then added
+//      to the matrix once per block while walking aT's three rows;
+//   termLocalPrime (6 symmetric values) to (bAdx[j], bAdx[i]): staged per pair for
+//      k_lp_gather (or added directly when no staging buffer is given);
+//   termLocal: each lane keeps innerLocal of its outer point over all pairs, and the block
+//      forms termLocal once at the end.
 template <int KER, typename T>
 __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
-    extern __shared__ int ldsc8[];
-    int* hk = ldsc8;                          // [HN] column nodes of aT's rows
-    int* hv = hk + a.HN;                      // [HN][3] their offsets in rows aAdx[0..2]
-    double* acc = (double*)(hv + 3 * a.HN);   // [HN][3] termNonloc (i, node) summed over pairs
+    extern __shared__ double ldsc8[];
+    double* acc = ldsc8;                 // [HN][3] termNonloc (i, node) summed over pairs
+    int* hk = (int*)(acc + 3 * a.HN);    // [HN] nodes of the cut partners (hash keys)
     __shared__ double sred[4][8];
+    __shared__ int full;                 // some node found no slot: entries go to the matrix directly
     const int aT = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -2019,18 +2141,23 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         hk[i] = -1;
         acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0;
     }
-    __syncthreads();
+    if (tid == 0) full = 0;
     long long rp[3];
+    int rlen[3];
     for (int j = 0; j < 3; j++) {
         rp[j] = a.rowptr[rA[j]];
-        const int len = (int)(a.rowptr[rA[j] + 1] - rp[j]);
-        for (int i = tid; i < len; i += 256) {
-            int sl = lds_slot(hk, a.HN, a.col[rp[j] + i]);
-            if (sl >= 0) hv[3 * sl + j] = i;
-            else atomicOr(&a.counters[CNT_ERROR], 16ull);
-        }
+        rlen[j] = (int)(a.rowptr[rA[j] + 1] - rp[j]);
     }
     __syncthreads();
+    // column nodes of the cut partners: every node of a partner lies in all three rows of aT
+    for (int p = tid; p < nB; p += 256) {
+        const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + p]
+                                    : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - p] & IDX_MASK);
+        for (int j = 0; j < 3; j++)
+            if (lds_slot_n(hk, a.HN, node_of(a.tri, b, j, a.is_DG)) < 0) full = 1;
+    }
+    __syncthreads();
+    const bool direct = full != 0;
     const bool kin = k < nPx;
     double x[2] = {0, 0};
     if (kin) toPhys(A.E, a.q.Px[2 * (kin ? k : 0)], a.q.Px[2 * (kin ? k : 0) + 1], x);
@@ -2103,11 +2230,23 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
         // the matrix once per block below
         if (k < 3) {
-            const int sl = lds_find(hk, a.HN, node_of(a.tri, b, k, a.is_DG));
-            if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
-            else
-                for (int i = 0; i < 3; i++)
-                    atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
+            const int node = node_of(a.tri, b, k, a.is_DG);
+            if (!direct) {
+                const int sl = lds_find_n(hk, a.HN, node);
+                if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
+                else
+                    for (int i = 0; i < 3; i++)
+                        atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
+            } else {
+                const bool lab = row_labelled(a.vlabel, node, a.is_DG);
+                for (int i = 0; i < 3; i++) {
+                    const double v = k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]);
+                    const int o = row_search(a.col + rp[i], rlen[i], node);
+                    if (o < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
+                    if (labA[i]) atomicAdd(&a.val[rp[i] + o], -v);
+                    if (lab) atomicAdd(&a.val[a.tpos[rp[i] + o]], -v);
+                }
+            }
         }
         // termLocalPrime: staged for k_lp_gather, or lane k adds entry (i, j) = k (and lane 0
         // also entry 8) to (bAdx[j], bAdx[i])
@@ -2137,18 +2276,19 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
     if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
     __syncthreads();
-    for (int sl = tid; sl < a.HN; sl += 256) {
-        const int node = hk[sl];
-        if (node < 0) continue;
-        const bool lab = row_labelled(a.vlabel, node, a.is_DG);
-        for (int i = 0; i < 3; i++) {
-            const double v = acc[3 * sl + i];
-            if (v == 0.0) continue;  // node not touched (its offsets may be unset)
-            const long long pos = rp[i] + hv[3 * sl + i];
-            if (labA[i]) atomicAdd(&a.val[pos], -v);
-            if (lab) atomicAdd(&a.val[a.tpos[pos]], -v);
-        }
-    }
+    // the accumulated columns reach the matrix: walk the three rows of aT, look each column
+    // node up in the partner-node table
+    if (!direct)
+        for (int i = 0; i < 3; i++)
+            for (int o = tid; o < rlen[i]; o += 256) {
+                const int node = a.col[rp[i] + o];
+                const int sl = lds_find_n(hk, a.HN, node);
+                if (sl < 0) continue;
+                const double v = acc[3 * sl + i];
+                if (v == 0.0) continue;
+                if (labA[i]) atomicAdd(&a.val[rp[i] + o], -v);
+                if (row_labelled(a.vlabel, node, a.is_DG)) atomicAdd(&a.val[a.tpos[rp[i] + o]], -v);
+            }
     if (tid < 9) {
         const int i = tid / 3, j = tid % 3;
         if (labA[i]) {
@@ -3197,6 +3337,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                          \
     } while (0)
     ev_begin(P, PH_CUT);
+    bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
     if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
@@ -3246,7 +3387,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     } while (0)
 #define LAUNCH_CUT8(KER)                                                                                  \
     do {                                                                                                  \
-        const size_t l8 = (size_t)nc.HN * 40;                                                             \
+        const size_t l8 = (size_t)nc.HN * 28;                                                             \
         if (c.precision == 1) {                                                                           \
             if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
             k_cut8<KER, float><<<c.nE, 256, l8, st>>>(nc);                                                \
@@ -3255,12 +3396,17 @@ void plan_assemble(Plan& P, const Estimates& est) {
             k_cut8<KER, double><<<c.nE, 256, l8, st>>>(nc);                                               \
         }                                                                                                 \
     } while (0)
-        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG && (size_t)nc.HN * 40 <= 160 * 1024) {
+        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG) {
+            long long ncut = 0;
+            HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            // table of the cut partners' nodes: about one new node per cut pair (the partners
+            // form an annulus), so 1.25 slots per average cut pair, rounded up to a power of two;
+            // a block whose nodes do not fit adds its entries to the matrix directly
+            nc.HN = (int)std::min<long long>(
+                4096, next_pow2(std::max<long long>(64, 5 * ncut / (4 * (long long)std::max(c.nE, 1)))));
             if (!use_sur) {
                 // termLocalPrime staged per cut pair, summed per inner element by k_lp_gather
-                long long ncut = 0;
-                HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
-                HIPCHECK(hipStreamSynchronize(st));
                 if (ncut <= INT_MAX) {  // the transposed cut entries hold int32 pair indices
                     if (P.stage_cap < ncut || !P.d_stage) {
                         dreserve(P, P.d_stage, (size_t)std::max<long long>(ncut, 1) * 6);
@@ -3276,11 +3422,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
                 case KER_FRACTIONAL: LAUNCH_CUT8(KER_FRACTIONAL); break;
                 default: LAUNCH_CUT8(KER_MICRO); break;
             }
-            if (nc.stage) {
-                HIPCHECK(hipGetLastError());
-                k_lp_gather<<<(c.nE + 3) / 4, 256, 0, st>>>(c.nE, c.is_DG, P.d_tri, P.d_vlabel, P.d_tptr, P.d_tcB,
-                                                             P.d_tcut, P.d_stage, P.d_posTT, P.d_val);
-            }
+            gather = nc.stage != nullptr;
         } else {
             switch (c.kernel) {
                 case KER_CONSTANT: LAUNCH_CUT(KER_CONSTANT, 1); break;
@@ -3295,6 +3437,12 @@ void plan_assemble(Plan& P, const Estimates& est) {
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_CUT);
+    ev_begin(P, PH_GATHER);
+    if (gather)
+        k_lp_gather<<<(c.nE + 3) / 4, 256, 0, st>>>(c.nE, c.is_DG, P.d_tri, P.d_vlabel, P.d_tptr, P.d_tcB, P.d_tcut,
+                                                     P.d_stage, P.d_posTT, P.d_val);
+    HIPCHECK(hipGetLastError());
+    ev_end(P, PH_GATHER);
 #undef DISPATCH_PAIRS
 #undef LAUNCH_PAIRS
 

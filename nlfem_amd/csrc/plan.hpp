@@ -34,7 +34,7 @@ struct PlanConfig {
 // Phase timings in milliseconds (hipEvent pairs on the plan's stream).
 enum Phase : int {
     PH_SETUP = 0, PH_DUAL, PH_BFS, PH_TRANSPOSE, PH_PATTERN, PH_POSITIONS, PH_CLOSE, PH_INTERIOR, PH_CUT,
-    PH_COMPACT, PH_COUNT
+    PH_COMPACT, PH_GATHER, PH_COUNT
 };
 
 struct Stats {

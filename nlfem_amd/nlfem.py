@@ -158,7 +158,7 @@ class Assembler:
 
     STAT_KEYS = ("pairs", "probes", "close", "interior", "cut", "bary", "nnz_nodes", "nnz", "cap", "max_list", "subtri",
                  "ms_dual", "ms_bfs", "ms_transpose", "ms_pattern", "ms_positions", "ms_close", "ms_interior",
-                 "ms_cut", "ms_compact", "sur_pairs", "sur_rejected", "sur_items")
+                 "ms_cut", "ms_compact", "ms_gather", "sur_pairs", "sur_rejected", "sur_items")
 
     def stats(self):
         out = np.zeros(len(self.STAT_KEYS))
