@@ -2069,37 +2069,6 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     }
 }
 
-// LDS hash of node ids with at most 64 probes (a full table reports -1 instead of scanning).
-NLFEM_DEV int lds_slot_n(int* h, int H, int key) {
-    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
-    for (int p = 0; p < 64 && p < H; p++) {
-        int old = atomicCAS(&h[s], -1, key);
-        if (old == -1 || old == key) return (int)s;
-        s = (s + 1) & (uint32_t)(H - 1);
-    }
-    return -1;
-}
-NLFEM_DEV int lds_find_n(const int* h, int H, int key) {
-    uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(H - 1);
-    for (int p = 0; p < 64 && p < H; p++) {
-        int k = h[s];
-        if (k == key) return (int)s;
-        if (k == -1) return -1;
-        s = (s + 1) & (uint32_t)(H - 1);
-    }
-    return -1;
-}
-// Offset of `node` in a sorted node-CSR row, or -1.
-NLFEM_DEV int row_search(const int* row, int len, int node) {
-    int lo = 0, hi = len;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (row[mid] < node) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < len && row[lo] == node) ? lo : -1;
-}
-
 // Cut pairs, scalar kernels with at most 8 outer points: one 256-thread block per outer
 // element; 32 groups of 8 lanes, one group per cut pair and one lane per outer point, so a
 // chunk of 32 pairs needs no block barrier.  Each lane retriangulates B(x_k, delta) n bT and
@@ -2119,9 +2088,9 @@ template <int KER, typename T>
 __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     extern __shared__ double ldsc8[];
     double* acc = ldsc8;                 // [HN][3] termNonloc (i, node) summed over pairs
-    int* hk = (int*)(acc + 3 * a.HN);    // [HN] nodes of the cut partners (hash keys)
+    int* hk = (int*)(acc + 3 * a.HN);    // [HN] column map of aT's three rows
     __shared__ double sred[4][8];
-    __shared__ int full;                 // some node found no slot: entries go to the matrix directly
+    __shared__ int scan_tmp[4];
     const int aT = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -2137,27 +2106,16 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         rA[j] = node_of(a.tri, aT, j, a.is_DG);
         labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
     }
-    for (int i = tid; i < a.HN; i += 256) {
-        hk[i] = -1;
-        acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0;
-    }
-    if (tid == 0) full = 0;
+    for (int i = tid; i < a.HN; i += 256) { acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0; }
     long long rp[3];
     int rlen[3];
     for (int j = 0; j < 3; j++) {
         rp[j] = a.rowptr[rA[j]];
         rlen[j] = (int)(a.rowptr[rA[j] + 1] - rp[j]);
     }
-    __syncthreads();
-    // column nodes of the cut partners: every node of a partner lies in all three rows of aT
-    for (int p = tid; p < nB; p += 256) {
-        const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + p]
-                                    : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - p] & IDX_MASK);
-        for (int j = 0; j < 3; j++)
-            if (lds_slot_n(hk, a.HN, node_of(a.tri, b, j, a.is_DG)) < 0) full = 1;
-    }
-    __syncthreads();
-    const bool direct = full != 0;
+    // every node of a cut partner lies in all three rows of aT
+    const bool all3[3] = {true, true, true};
+    const ColMap cm = colmap_build(hk, a.HN, a.col, rp, rlen, all3, scan_tmp, a.counters);
     const bool kin = k < nPx;
     double x[2] = {0, 0};
     if (kin) toPhys(A.E, a.q.Px[2 * (kin ? k : 0)], a.q.Px[2 * (kin ? k : 0) + 1], x);
@@ -2230,23 +2188,11 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
         // the matrix once per block below
         if (k < 3) {
-            const int node = node_of(a.tri, b, k, a.is_DG);
-            if (!direct) {
-                const int sl = lds_find_n(hk, a.HN, node);
-                if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
-                else
-                    for (int i = 0; i < 3; i++)
-                        atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
-            } else {
-                const bool lab = row_labelled(a.vlabel, node, a.is_DG);
-                for (int i = 0; i < 3; i++) {
-                    const double v = k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]);
-                    const int o = row_search(a.col + rp[i], rlen[i], node);
-                    if (o < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
-                    if (labA[i]) atomicAdd(&a.val[rp[i] + o], -v);
-                    if (lab) atomicAdd(&a.val[a.tpos[rp[i] + o]], -v);
-                }
-            }
+            const int sl = cm.find(node_of(a.tri, b, k, a.is_DG));
+            if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
+            else
+                for (int i = 0; i < 3; i++)
+                    atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
         }
         // termLocalPrime: staged for k_lp_gather, or lane k adds entry (i, j) = k (and lane 0
         // also entry 8) to (bAdx[j], bAdx[i])
@@ -2276,19 +2222,17 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
     for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
     if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
     __syncthreads();
-    // the accumulated columns reach the matrix: walk the three rows of aT, look each column
-    // node up in the partner-node table
-    if (!direct)
-        for (int i = 0; i < 3; i++)
-            for (int o = tid; o < rlen[i]; o += 256) {
-                const int node = a.col[rp[i] + o];
-                const int sl = lds_find_n(hk, a.HN, node);
-                if (sl < 0) continue;
-                const double v = acc[3 * sl + i];
-                if (v == 0.0) continue;
-                if (labA[i]) atomicAdd(&a.val[rp[i] + o], -v);
-                if (row_labelled(a.vlabel, node, a.is_DG)) atomicAdd(&a.val[a.tpos[rp[i] + o]], -v);
-            }
+    // the accumulated columns reach the matrix: walk the three rows of aT
+    for (int i = 0; i < 3; i++)
+        for (int o = tid; o < rlen[i]; o += 256) {
+            const int node = a.col[rp[i] + o];
+            const int sl = cm.find(node);
+            if (sl < 0) continue;
+            const double v = acc[3 * sl + i];
+            if (v == 0.0) continue;
+            if (labA[i]) atomicAdd(&a.val[rp[i] + o], -v);
+            if (row_labelled(a.vlabel, node, a.is_DG)) atomicAdd(&a.val[a.tpos[rp[i] + o]], -v);
+        }
     if (tid < 9) {
         const int i = tid / 3, j = tid % 3;
         if (labA[i]) {
@@ -3400,11 +3344,6 @@ void plan_assemble(Plan& P, const Estimates& est) {
             long long ncut = 0;
             HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
-            // table of the cut partners' nodes: about one new node per cut pair (the partners
-            // form an annulus), so 1.25 slots per average cut pair, rounded up to a power of two;
-            // a block whose nodes do not fit adds its entries to the matrix directly
-            nc.HN = (int)std::min<long long>(
-                4096, next_pow2(std::max<long long>(64, 5 * ncut / (4 * (long long)std::max(c.nE, 1)))));
             if (!use_sur) {
                 // termLocalPrime staged per cut pair, summed per inner element by k_lp_gather
                 if (ncut <= INT_MAX) {  // the transposed cut entries hold int32 pair indices
