@@ -147,6 +147,26 @@ NLFEM_DEV int lds_find(const int* h, int H, int key) {
     return -1;
 }
 
+// Exclusive prefix sum over a 256-thread block (tmp: 4 ints of LDS); total returned in *tot.
+NLFEM_DEV int block_exscan256(int v, int* tmp, int* tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int off = 0, all = 0;
+    for (int i = 0; i < 4; i++) {
+        if (i < w) off += tmp[i];
+        all += tmp[i];
+    }
+    __syncthreads();
+    *tot = all;
+    return off + x - v;
+}
+
 // Node (degree of freedom) j of element T: the vertex for CG, 3T+j for DG.
 NLFEM_DEV int node_of(const int* tri, int T, int j, int is_DG) { return is_DG ? 3 * T + j : tri[3 * T + j]; }
 NLFEM_DEV bool row_labelled(const int* vlabel, int node, int is_DG) { return is_DG || vlabel[node] > 0; }
@@ -348,14 +368,23 @@ NLFEM_DEV int classify_quick(const BfsArgs& a, const OuterPts& op, const ElemRec
     return CLS_HARD;
 }
 
-// Squared distance from (x0, x1) to the segment P -> Q.
-NLFEM_DEV double seg_sqdist(double x0, double x1, double P0, double P1, double Q0, double Q1) {
+// Whether the segment P -> Q stays farther than sqrt(sqf) from (x0, x1), without a division:
+// with w = x - P, D = Q - P and t = w.D, the squared distance is |w|^2 (t <= 0), |x - Q|^2
+// (t >= |D|^2), or (|w|^2 |D|^2 - t^2) / |D|^2, compared as (|w|^2 |D|^2 - t^2) > sqf |D|^2.
+// Used as a conservative skip with sqf = delta^2 (1 + 1e-9): its rounding (relative 1e-16
+// times (|w| / distance)^2, about 1e-16 near the threshold) is far below the margin.
+NLFEM_DEV bool seg_far(double x0, double x1, double P0, double P1, double Q0, double Q1, double sqf) {
     const double D0 = Q0 - P0, D1 = Q1 - P1;
-    const double dd = D0 * D0 + D1 * D1;
-    double t = dd > 0 ? ((x0 - P0) * D0 + (x1 - P1) * D1) / dd : 0.0;
-    t = t < 0 ? 0 : (t > 1 ? 1 : t);
-    const double c0 = P0 + t * D0 - x0, c1 = P1 + t * D1 - x1;
-    return c0 * c0 + c1 * c1;
+    const double w0 = x0 - P0, w1 = x1 - P1;
+    const double t = w0 * D0 + w1 * D1, dd = D0 * D0 + D1 * D1, ww = w0 * w0 + w1 * w1;
+    const double u0 = x0 - Q0, u1 = x1 - Q1;
+    const double uu = u0 * u0 + u1 * u1;
+    return t <= 0 ? ww > sqf : (t >= dd ? uu > sqf : ww * dd - t * t > sqf * dd);
+}
+// All three edges of the triangle E stay farther than sqrt(sqf) from (x0, x1).
+NLFEM_DEV bool tri_far(double x0, double x1, const double* E, double sqf) {
+    return seg_far(x0, x1, E[0], E[1], E[2], E[3], sqf) && seg_far(x0, x1, E[2], E[3], E[4], E[5], sqf) &&
+           seg_far(x0, x1, E[4], E[5], E[0], E[1], sqf);
 }
 
 // The reference predicate (some outer point's polygon has >= 3 points) by retriangulation.
@@ -364,10 +393,7 @@ NLFEM_DEV double seg_sqdist(double x0, double x1, double P0, double P1, double Q
 NLFEM_DEV int classify_full(const BfsArgs& a, const OuterPts& op, const ElemRec& B) {
     for (int k = 0; k < a.q.nPx; k++) {
         const double x0 = op.x[2 * k], x1 = op.x[2 * k + 1];
-        const double dmin = fmin(seg_sqdist(x0, x1, B.E[0], B.E[1], B.E[2], B.E[3]),
-                                 fmin(seg_sqdist(x0, x1, B.E[2], B.E[3], B.E[4], B.E[5]),
-                                      seg_sqdist(x0, x1, B.E[4], B.E[5], B.E[0], B.E[1])));
-        if (dmin > a.sq_far) continue;
+        if (tri_far(x0, x1, B.E, a.sq_far)) continue;
         NoEmit ne;
         if (retriangulate<true>(x0, x1, B.E, a.sqdelta, a.caps != 0, ne) >= 3) return (int)CLS_CUT;
     }
@@ -813,6 +839,60 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
         __syncthreads();
         elist = el;
     }
+    // node set.  CG rows whose nodes span at most 32 DH indices: a bitmap of that window in the
+    // DH words, compacted in index order, so the row comes out sorted; otherwise the LDS hash.
+    __shared__ int nlo, nhi;
+    if (tid == 0) { nlo = INT_MAX; nhi = -1; }
+    __syncthreads();
+    if (!a.is_DG) {
+        int mylo = INT_MAX, myhi = -1;
+        for (int i = tid; i < nslots; i += 256) {
+            const int U = elist[i];
+            if (U < 0) continue;
+            for (int j = 0; j < 3; j++) {
+                const int d = a.tri[3 * U + j];
+                mylo = min(mylo, d);
+                myhi = max(myhi, d);
+            }
+        }
+        for (int m = 32; m > 0; m >>= 1) {
+            mylo = min(mylo, __shfl_xor(mylo, m, 64));
+            myhi = max(myhi, __shfl_xor(myhi, m, 64));
+        }
+        if (lane == 0) { atomicMin(&nlo, mylo); atomicMax(&nhi, myhi); }
+    }
+    __syncthreads();
+    const long long nspan = (long long)nhi - nlo + 1;
+    if (!a.is_DG && nhi >= nlo && nspan <= 32LL * a.DH) {
+        const int nw = (int)((nspan + 31) / 32);
+        for (int i = tid; i < nw; i += 256) dh[i] = 0;
+        __syncthreads();
+        for (int i = tid; i < nslots; i += 256) {
+            const int U = elist[i];
+            if (U < 0) continue;
+            for (int j = 0; j < 3; j++) {
+                const unsigned d = (unsigned)(a.tri[3 * U + j] - nlo);
+                atomicOr((unsigned*)&dh[d >> 5], 1u << (d & 31));
+            }
+        }
+        __syncthreads();
+        const int per = (nw + 255) / 256, w0 = min(tid * per, nw), w1 = min(w0 + per, nw);
+        int c = 0;
+        for (int i = w0; i < w1; i++) c += __popc((unsigned)dh[i]);
+        __shared__ int stmp[4];
+        int tot;
+        int q = block_exscan256(c, stmp, &tot);
+        for (int i = w0; i < w1; i++) {
+            unsigned bits = (unsigned)dh[i];
+            while (bits) {
+                const int b = __ffs(bits) - 1;
+                bits &= bits - 1;
+                if (q < a.RC) row[q] = nlo + 32 * i + b;
+                q++;
+            }
+        }
+        if (tid == 0) nDof = tot;
+    } else
     for (int i = tid; i < nslots; i += 256) {  // a multiple of 256: uniform trip count
         const int U = elist[i];
         for (int j = 0; j < 3; j++) {
@@ -856,6 +936,17 @@ __global__ __launch_bounds__(256) void k_pattern_sort(PatArgs a) {
     const long long o = a.rowptr[r];
     const int n = (int)(a.rowptr[r + 1] - o);
     const int* row = a.tmp + (size_t)r * a.RC;
+    // rows built from a node bitmap are sorted already
+    __shared__ int unsorted;
+    if (tid == 0) unsorted = 0;
+    __syncthreads();
+    for (int i = tid; i + 1 < n; i += 256)
+        if (row[i] > row[i + 1]) unsorted = 1;
+    __syncthreads();
+    if (!unsorted) {
+        for (int i = tid; i < n; i += 256) a.col[o + i] = row[i];
+        return;
+    }
     int m = 1;
     while (m < n) m <<= 1;
     for (int i = tid; i < m; i += 256) sb[i] = i < n ? row[i] : 0x7fffffff;
@@ -1391,26 +1482,6 @@ __global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
 // around each node.  No global atomics.
 constexpr unsigned long long ERR_HASH = 128ull;
 
-// Exclusive prefix sum over a 256-thread block (tmp: 4 ints of LDS); total returned in *tot.
-NLFEM_DEV int block_exscan256(int v, int* tmp, int* tot) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) tmp[w] = x;
-    __syncthreads();
-    int off = 0, all = 0;
-    for (int i = 0; i < 4; i++) {
-        if (i < w) off += tmp[i];
-        all += tmp[i];
-    }
-    __syncthreads();
-    *tot = all;
-    return off + x - v;
-}
-
 // Column map of an outer element's node rows: node id -> index into the block's column
 // accumulators (< H).  Rows are sorted, so their nodes lie in [first, last] of the rows.
 // Rank mode, when that window spans at most 16 H nodes: a bitmap of the window (WW words)
@@ -1937,10 +2008,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             double x[2];
             toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
             int nt = 0;
-            const double dmin = fmin(seg_sqdist(x[0], x[1], B.E[0], B.E[1], B.E[2], B.E[3]),
-                                     fmin(seg_sqdist(x[0], x[1], B.E[2], B.E[3], B.E[4], B.E[5]),
-                                          seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
-            if (!(dmin > a.sq_far)) {
+            if (!tri_far(x[0], x[1], B.E, a.sq_far)) {
                 if constexpr (KER == KER_CONSTANT && OD == 1) {  // integrated here: no fan triangles for phase B
                     ConstantItem ci(a, lus[p], B.E);
                     retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, ci);
@@ -2134,10 +2202,7 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
         const ElemRec B = a.elem[b];
         double inL = 0, inN[3] = {0, 0, 0}, sLP[6] = {0, 0, 0, 0, 0, 0};
         if (kin) {
-            const double dmin = fmin(seg_sqdist(x[0], x[1], B.E[0], B.E[1], B.E[2], B.E[3]),
-                                     fmin(seg_sqdist(x[0], x[1], B.E[2], B.E[3], B.E[4], B.E[5]),
-                                          seg_sqdist(x[0], x[1], B.E[4], B.E[5], B.E[0], B.E[1])));
-            if (!(dmin > a.sq_far)) {
+            if (!tri_far(x[0], x[1], B.E, a.sq_far)) {
                 const PivotLU lu = pivot_lu(B.E);
                 if constexpr (KER == KER_CONSTANT) {
                     ConstantItem ci(a, lu, B.E);
