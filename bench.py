@@ -48,10 +48,13 @@ def algorithmic_flops(st, nPx, nPy):
     return f_int, f_cut
 
 
-def conf_uses_cut8(conf, kernel):
-    """The library's cut-pair kernel for scalar kernels with <= 8 outer points and CG."""
-    return (kernel.get("outputdim", 1) == 1 and conf["quadrature"]["outer"]["points"].shape[0] <= 8
-            and conf.get("ansatz", "CG") == "CG")
+def cut_kernel_name(conf, kernel):
+    """The library's cut-pair kernel: k_cut8 for scalar kernels with <= 8 outer points and CG
+    (its 128-register instance k_cut8_o4 for the constant and parabola kernels), else k_cut."""
+    if not (kernel.get("outputdim", 1) == 1 and conf["quadrature"]["outer"]["points"].shape[0] <= 8
+            and conf.get("ansatz", "CG") == "CG"):
+        return "k_cut"
+    return "k_cut8_o4" if kernel["function"] in ("constant", "parabola") else "k_cut8"
 
 
 def measured_traffic(kernel_name, workload):
@@ -149,7 +152,7 @@ def main():
     t_dom = ms_phase["ms_" + dom] / 1000.0
     achieved = f_dom / t_dom / 1e12
     roof = {"bound": "mfma", "compute": "fp64 (VALU; FP64 vector and matrix peaks coincide on MI355X)",
-            "kernel": ("k_cut8" if conf_uses_cut8(conf, kernel) else "k_cut") if dom == "cut" else "k_interior",
+            "kernel": cut_kernel_name(conf, kernel) if dom == "cut" else "k_interior",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": None, "flops_per_launch": f_dom, "ms_per_launch": round(t_dom * 1000, 4)}

@@ -2153,7 +2153,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
 //   termLocal: each lane keeps innerLocal of its outer point over all pairs, and the block
 //      forms termLocal once at the end.
 template <int KER, typename T>
-__global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
+__device__ __forceinline__ void cut8_body(const NumArgs& a) {
     extern __shared__ double ldsc8[];
     double* acc = ldsc8;                 // [HN][3] termNonloc (i, node) summed over pairs
     int* hk = (int*)(acc + 3 * a.HN);    // [HN] column map of aT's three rows
@@ -2309,6 +2309,19 @@ __global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
             atomicAdd(&a.val[a.posTT[9 * aT + 3 * i + j]], v);
         }
     }
+}
+
+// The kernels around cut8_body.  Its register demand (about 160 VGPRs unconstrained) allows
+// 3 wavefronts per SIMD; capped at 128 for 4 wavefronts, the constant and parabola instances
+// spill 2 and 12 registers and run 10 % and 4 % faster, while the fractional kernel's would
+// spill ~200.
+template <int KER, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_cut8_o4(NumArgs a) {
+    cut8_body<KER, T>(a);
+}
+template <int KER, typename T>
+__global__ __launch_bounds__(256) void k_cut8(NumArgs a) {
+    cut8_body<KER, T>(a);
 }
 
 // termLocalPrime of the cut pairs staged by k_cut8, summed per inner element: one wavefront
@@ -3394,17 +3407,18 @@ void plan_assemble(Plan& P, const Estimates& est) {
             k_cut<KER, OD, double><<<c.nE, 256, lb, st>>>(nc);                                                       \
         }                                                                                                            \
     } while (0)
-#define LAUNCH_CUT8(KER)                                                                                  \
+#define LAUNCH_CUT8_K(KFN, KER)                                                                           \
     do {                                                                                                  \
         const size_t l8 = (size_t)nc.HN * 28;                                                             \
         if (c.precision == 1) {                                                                           \
-            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
-            k_cut8<KER, float><<<c.nE, 256, l8, st>>>(nc);                                                \
+            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)KFN<KER, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
+            KFN<KER, float><<<c.nE, 256, l8, st>>>(nc);                                                   \
         } else {                                                                                          \
-            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut8<KER, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
-            k_cut8<KER, double><<<c.nE, 256, l8, st>>>(nc);                                               \
+            if (l8 > 64 * 1024) (void)hipFuncSetAttribute((const void*)KFN<KER, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8); \
+            KFN<KER, double><<<c.nE, 256, l8, st>>>(nc);                                                  \
         }                                                                                                 \
     } while (0)
+#define LAUNCH_CUT8(KER) LAUNCH_CUT8_K(k_cut8, KER)
         if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG) {
             long long ncut = 0;
             HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -3421,8 +3435,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
                 }
             }
             switch (c.kernel) {
-                case KER_CONSTANT: LAUNCH_CUT8(KER_CONSTANT); break;
-                case KER_PARABOLA: LAUNCH_CUT8(KER_PARABOLA); break;
+                case KER_CONSTANT: LAUNCH_CUT8_K(k_cut8_o4, KER_CONSTANT); break;
+                case KER_PARABOLA: LAUNCH_CUT8_K(k_cut8_o4, KER_PARABOLA); break;
                 case KER_FRACTIONAL: LAUNCH_CUT8(KER_FRACTIONAL); break;
                 default: LAUNCH_CUT8(KER_MICRO); break;
             }
@@ -3438,6 +3452,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }
 #undef LAUNCH_CUT
 #undef LAUNCH_CUT8
+#undef LAUNCH_CUT8_K
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_CUT);
