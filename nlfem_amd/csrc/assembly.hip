@@ -2138,12 +2138,12 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
 }
 
 // Cut pairs, scalar kernels with at most 8 outer points: one 256-thread block per outer
-// element; 32 groups of 8 lanes, one group per cut pair and one lane per outer point, so a
-// chunk of 32 pairs needs no block barrier.  Each lane retriangulates B(x_k, delta) n bT and
+// element; groups of 8 or nPx lanes, one group per cut pair and one lane per outer point, so
+// a chunk of pairs needs no block barrier.  Each lane retriangulates B(x_k, delta) n bT and
 // integrates the fan triangles as they are emitted (constant kernel: ConstantItem's
 // geometric moments; otherwise the inner rule point by point, as the reference's
 // integrate_retriangulate), forms its outer point's share of termNonloc and termLocalPrime
-// (src/integration.cpp:508-537), and the group sums them with three xor shuffles.  Where the
+// (src/integration.cpp:508-537), and the group sums them in three shuffle steps.  Where the
 // blocks go (src/Cassemble.cpp:721-770):
 //   -termNonloc (i, j) to (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]): summed per column node
 //      bAdx[j] in LDS over all pairs of the block (a table of the partners' nodes), then added
@@ -2157,17 +2157,27 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
     extern __shared__ double ldsc8[];
     double* acc = ldsc8;                 // [HN][3] termNonloc (i, node) summed over pairs
     int* hk = (int*)(acc + 3 * a.HN);    // [HN] column map of aT's three rows
-    __shared__ double sred[4][8];
+    __shared__ double sred[256];
     __shared__ int scan_tmp[4];
     const int aT = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
-    const int g = tid >> 3, k = tid & 7;
     const ElemRec A = a.elem[aT];
     if (A.label == 0) return;
     const int nB = a.sur.list_in ? a.sur.rcnt[aT] : a.cntB[aT];
     if (nB == 0) return;
     const int nPx = a.q.nPx;
+    // groups of GS lanes, one per cut pair, one lane per outer point.  The constant kernel's
+    // items are cheap and its groups are 8 lanes wide (a power of two: the sums are
+    // all-reduced with xor shuffles and the scatter is shared by the lanes); the other
+    // kernels' items dominate, and groups of nPx lanes (9 per wavefront for the 7-point rule)
+    // leave fewer lanes idle (tree sum into the group's first lane, which scatters).
+    constexpr bool G8 = KER == KER_CONSTANT;
+    const int GS = G8 ? 8 : nPx;
+    const int G = 64 / GS;
+    const int gl = lane / GS, k = lane - gl * GS;
+    const bool lane_on = gl < G;
+    const int g = w * G + gl;
     int rA[3];
     bool labA[3];
     for (int j = 0; j < 3; j++) {
@@ -2184,7 +2194,7 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
     // every node of a cut partner lies in all three rows of aT
     const bool all3[3] = {true, true, true};
     const ColMap cm = colmap_build(hk, a.HN, a.col, rp, rlen, all3, scan_tmp, a.counters);
-    const bool kin = k < nPx;
+    const bool kin = lane_on && k < nPx;
     double x[2] = {0, 0};
     if (kin) toPhys(A.E, a.q.Px[2 * (kin ? k : 0)], a.q.Px[2 * (kin ? k : 0) + 1], x);
     const double ak = kin ? A.absDet * a.q.dx[k] : 0.0;   // |aT| dx_k
@@ -2194,9 +2204,9 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
     double sumL = 0;  // innerLocal of this lane's outer point over its pairs
     unsigned mytri = 0;
     double* const stage = a.sur.list_in ? nullptr : a.stage;
-    for (int c0 = 0; c0 < nB; c0 += 32) {
+    for (int c0 = 0; c0 < nB; c0 += 4 * G) {
         const int p = c0 + g;
-        if (p >= nB) break;  // whole groups leave together
+        if (!lane_on || p >= nB) break;  // whole groups leave together
         const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + p]
                                     : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - p] & IDX_MASK);
         const ElemRec B = a.elem[b];
@@ -2242,47 +2252,80 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
                 for (int j = 0; j < 3; j++) tN[3 * i + j] = ak * psx[i] * inN[j];
             for (int m = 0; m < 6; m++) tLP[m] = wk * sLP[m];
         }
+        if constexpr (G8) {
 #pragma unroll
-        for (int m = 1; m < 8; m <<= 1) {
+            for (int m = 1; m < 8; m <<= 1) {
 #pragma unroll
-            for (int e = 0; e < 9; e++) tN[e] += shfl_xor_d(tN[e], m);
+                for (int e = 0; e < 9; e++) tN[e] += shfl_xor_d(tN[e], m);
 #pragma unroll
-            for (int e = 0; e < 6; e++) tLP[e] += shfl_xor_d(tLP[e], m);
-        }
-        // -termNonloc (i, j) goes to the entries (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]):
-        // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
-        // the matrix once per block below
-        if (k < 3) {
-            const int sl = cm.find(node_of(a.tri, b, k, a.is_DG));
-            if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
-            else
-                for (int i = 0; i < 3; i++)
-                    atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
-        }
-        // termLocalPrime: staged for k_lp_gather, or lane k adds entry (i, j) = k (and lane 0
-        // also entry 8) to (bAdx[j], bAdx[i])
-        if (stage) {
-            if (k < 6) {
-                double v = tLP[0];
+                for (int e = 0; e < 6; e++) tLP[e] += shfl_xor_d(tLP[e], m);
+            }
+            // -termNonloc (i, j) goes to the entries (aAdx[i], bAdx[j]) and (bAdx[j], aAdx[i]):
+            // lane j < 3 adds column j of this pair to node bAdx[j]'s accumulators, which reach
+            // the matrix once per block below
+            if (k < 3) {
+                const int sl = cm.find(node_of(a.tri, b, k, a.is_DG));
+                if (sl < 0) atomicOr(&a.counters[CNT_ERROR], 8ull);
+                else
+                    for (int i = 0; i < 3; i++)
+                        atomicAdd(&acc[3 * sl + i], k == 0 ? tN[3 * i] : (k == 1 ? tN[3 * i + 1] : tN[3 * i + 2]));
+            }
+            // termLocalPrime: staged for k_lp_gather, or lane k adds entry (i, j) = k (and lane 0
+            // also entry 8) to (bAdx[j], bAdx[i])
+            if (stage) {
+                if (k < 6) {
+                    double v = tLP[0];
 #pragma unroll
-                for (int m = 1; m < 6; m++) v = k == m ? tLP[m] : v;
-                stage[(a.coff[aT] + p) * 6 + k] = v;
+                    for (int m = 1; m < 6; m++) v = k == m ? tLP[m] : v;
+                    stage[(a.coff[aT] + p) * 6 + k] = v;
+                }
+            } else {
+                for (int e = k; e < 9; e += 8) {
+                    const int i = e / 3, j = e % 3;
+                    double v = tLP[0];
+#pragma unroll
+                    for (int m = 1; m < 6; m++) v = sym6(i, j) == m ? tLP[m] : v;
+                    if (row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG))
+                        atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], v);
+                }
             }
         } else {
-            for (int e = k; e < 9; e += 8) {
-                const int i = e / 3, j = e % 3;
-                double v = tLP[0];
+            // tree sum over the group's lanes into its first lane (partners k + 1, k + 2, k + 4)
 #pragma unroll
-                for (int m = 1; m < 6; m++) v = sym6(i, j) == m ? tLP[m] : v;
-                if (row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG))
-                    atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], v);
+            for (int d = 1; d < 8; d <<= 1) {
+                const bool take = (k & (2 * d - 1)) == 0 && k + d < GS;
+                const int src = min(lane + d, 63);
+#pragma unroll
+                for (int e = 0; e < 9; e++) {
+                    const double o = __shfl(tN[e], src, 64);
+                    if (take) tN[e] += o;
+                }
+#pragma unroll
+                for (int e = 0; e < 6; e++) {
+                    const double o = __shfl(tLP[e], src, 64);
+                    if (take) tLP[e] += o;
+                }
+            }
+            if (k == 0) {
+                for (int j = 0; j < 3; j++) {
+                    const int sl = cm.find(node_of(a.tri, b, j, a.is_DG));
+                    if (sl < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
+                    for (int i = 0; i < 3; i++) atomicAdd(&acc[3 * sl + i], tN[3 * i + j]);
+                }
+                if (stage) {
+                    for (int m = 0; m < 6; m++) stage[(a.coff[aT] + p) * 6 + m] = tLP[m];
+                } else {
+                    for (int j = 0; j < 3; j++) {
+                        if (!row_labelled(a.vlabel, node_of(a.tri, b, j, a.is_DG), a.is_DG)) continue;
+                        for (int i = 0; i < 3; i++) atomicAdd(&a.val[a.posTT[9 * b + 3 * j + i]], tLP[sym6(i, j)]);
+                    }
+                }
             }
         }
     }
-    // termLocal[i][j] = sum_k |aT| dx_k psi_i(x_k) psi_j(x_k) innerLocal_k: sum over the lanes of
-    // one outer point (k = tid & 7), then over the waves
-    for (int m = 8; m < 64; m <<= 1) sumL += shfl_xor_d(sumL, m);
-    if (lane < 8) sred[w][lane] = sumL;
+    // termLocal[i][j] = sum_k |aT| dx_k psi_i(x_k) psi_j(x_k) innerLocal_k: innerLocal_k summed
+    // over the lanes of outer point k below
+    sred[tid] = kin ? sumL : 0.0;
     unsigned tri = mytri;
     for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
     if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
@@ -2303,7 +2346,9 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
         if (labA[i]) {
             double v = 0;
             for (int kk = 0; kk < nPx; kk++) {
-                const double S = sred[0][kk] + sred[1][kk] + sred[2][kk] + sred[3][kk];
+                double S = 0;
+                for (int ww = 0; ww < 4; ww++)
+                    for (int q = 0; q < G; q++) S += sred[64 * ww + q * GS + kk];
                 v += A.absDet * a.q.dx[kk] * a.q.psix[3 * kk + i] * a.q.psix[3 * kk + j] * S;
             }
             atomicAdd(&a.val[a.posTT[9 * aT + 3 * i + j]], v);
