@@ -984,13 +984,20 @@ __global__ void k_row_of(int nNodes, const long long* rowptr, int* rowOf) {
     int r = blockIdx.x;
     for (long long p = rowptr[r] + threadIdx.x; p < rowptr[r + 1]; p += blockDim.x) rowOf[p] = r;
 }
+// The pattern is symmetric: entry p = (r, c) with r < c finds its transpose q = (c, r) by a
+// binary search in row c and sets both tpos[p] = q and tpos[q] = p; diagonal entries are
+// their own transpose, and entries below the diagonal are set by their partner.
 __global__ void k_tpos(long long nnz, const long long* rowptr, const int* col, const int* rowOf, int* tpos,
                        unsigned long long* counters) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nnz) return;
-    long long q = find_pos(rowptr, col, col[p], rowOf[p]);
-    if (q < 0) atomicOr(&counters[CNT_ERROR], 4ull);
+    const int r = rowOf[p], c = col[p];
+    if (r > c) return;
+    if (r == c) { tpos[p] = (int)p; return; }
+    long long q = find_pos(rowptr, col, c, r);
+    if (q < 0) { atomicOr(&counters[CNT_ERROR], 4ull); tpos[p] = -1; return; }
     tpos[p] = (int)q;
+    tpos[q] = (int)p;
 }
 __global__ void k_posTT(int nE, const int* tri, int is_DG, const long long* rowptr, const int* col, int* posTT,
                         unsigned long long* counters) {
