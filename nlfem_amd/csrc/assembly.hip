@@ -685,7 +685,9 @@ NLFEM_DEV int lds_find_b(const int* h, int key) {
 }
 
 // FILL = false: tcnt[b] += entries of row b.  FILL = true: place the entries; tfill / tfillB
-// count the front / back entries placed so far (tfillB[b] ends as row b's cut-entry count).
+// count the front / back entries placed so far (they end as row b's front and cut-entry
+// counts).  Row b occupies [tptr[b], tptr[b+1]): the exact CSR rows of a counting pass, or
+// padded rows of a fixed capacity, whose overflow k_tcheck reports.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
                                                    const int* cntB, int* tcnt, const long long* tptr, int* tfill,
@@ -723,15 +725,31 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
             const int b = (int)(e & IDX_MASK);
             const int sl = lds_find_b(hk, b);
             long long q;
+            const long long room = tptr[b + 1] - tptr[b];  // the row's capacity
             if (p < nF) {
-                q = tptr[b] + (sl >= 0 ? atomicAdd(&hf[sl], 1) : atomicAdd(&tfill[b], 1));
+                const long long r = sl >= 0 ? atomicAdd(&hf[sl], 1) : atomicAdd(&tfill[b], 1);
+                if (r >= room) continue;  // padded rows: overflow, detected by k_tcheck
+                q = tptr[b] + r;
             } else {
-                q = tptr[b + 1] - 1 - (sl >= 0 ? atomicAdd(&hb[sl], 1) : atomicAdd(&tfillB[b], 1));
+                const long long r = sl >= 0 ? atomicAdd(&hb[sl], 1) : atomicAdd(&tfillB[b], 1);
+                if (r >= room) continue;
+                q = tptr[b + 1] - 1 - r;
                 tcut[q] = (int)(coff[aT] + (p - nF));
             }
             tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
         }
     }
+}
+
+// Padded transposed rows: row b starts at b * capT.
+__global__ void k_tptr_padded(int nE, int capT, long long* tptr) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= nE) tptr[b] = (long long)b * capT;
+}
+// A padded row whose front and back entries do not fit its capacity: overflow flag 16.
+__global__ void k_tcheck(int nE, int capT, const int* tF, const int* tB, unsigned long long* counters) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nE && (long long)tF[b] + tB[b] > capT) atomicOr(&counters[CNT_OVERFLOW], 16ull);
 }
 
 // ======================================================================= pattern
@@ -745,6 +763,8 @@ struct PatArgs {
     const int* cntB;
     const long long* tptr;
     const int* tadj;
+    const int* tF;   // front (interior, close) entries of each transposed row
+    const int* tB;   // back (cut) entries of each transposed row
     int EH, DH, SB;  // element hash, dof hash, sort buffer sizes (powers of two)
     int* tmp;        // unsorted rows, RC entries each
     int RC;
@@ -792,12 +812,15 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
         const int T = a.is_DG ? (int)ti : a.vel[ti];
         const int nF = a.cntF[T], n = nF + a.cntB[T];
         const long long tb = a.tptr[T], te = a.tptr[T + 1];
-        const int nt = (int)(te - tb);
+        const int ntF = a.tF[T], nt = ntF + a.tB[T];
         if (n + nt > 0 && tid == 0) einsert(T);
         for (int p = tid; p < n + nt; p += 256) {
             int U;
             if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
-            else U = (int)((uint32_t)a.tadj[tb + (p - n)] & IDX_MASK);
+            else {
+                const int e = p - n;
+                U = (int)((uint32_t)a.tadj[e < ntF ? tb + e : te - 1 - (e - ntF)] & IDX_MASK);
+            }
             einsert(U);
         }
     }
@@ -1058,6 +1081,7 @@ struct NumArgs {
     // interior pairs, owner computes: inverse relation and per-(element, local vertex) partial rows
     const long long* tptr;
     const int* tadj;
+    const int* tF;  // front entries of each transposed row (interior and close pairs)
     const long long* poff;
     double* P;
     SurArgs sur;
@@ -1607,7 +1631,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                 atomicAdd(&acc[s * 3 + 0], w * a.q1[j]);
             }
         }
-        const long long tb = a.tptr[T0], te = a.tptr[T0 + 1];
+        const long long tb = a.tptr[T0], te = tb + a.tF[T0];  // interior entries are front entries
         for (long long q = tb + tid; q < te; q += 256) {
             const uint32_t e = (uint32_t)a.tadj[q];
             if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
@@ -1649,7 +1673,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         }
     }
     // T as inner element: pairs (U, T) with T in the interior front segment of U
-    const long long tb = a.tptr[T0], te = FACT ? a.tptr[T0] : a.tptr[T0 + 1];
+    const long long tb = a.tptr[T0], te = FACT ? tb : tb + a.tF[T0];
     for (long long q = tb + tid; q < te; q += 256) {
         const uint32_t e = (uint32_t)a.tadj[q];
         if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
@@ -3206,23 +3230,49 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // 1400 entries keep the LDS hash under half full
     const int tg = (use_bm && P.stats.max_list <= 1400) ? 8 : 1;
     const int tblocks = (c.nE + tg - 1) / tg;
-    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
-    k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
-                                                nullptr, nullptr, nullptr, nullptr, nullptr);
     dreserve(P, P.d_tptr, (size_t)c.nE + 1);
-    scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
-    long long ntadj = 0;
-    HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    dreserve(P, P.d_tadj, (size_t)ntadj);
-    dreserve(P, P.d_tcut, (size_t)ntadj);
     dreserve(P, P.d_coff, (size_t)c.nE + 1);
     dreserve(P, P.d_tcB, (size_t)c.nE);
+    dreserve(P, P.d_tF, (size_t)c.nE);
     scan_exclusive<int>(P.d_cntB, P.d_coff, c.nE, scan_scratch(P), st);
-    HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
-    HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
-    k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
-                                               d_cnt, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
+    // Padded rows of the neighbour lists' capacity first (the relation is nearly symmetric, so
+    // a transposed row rarely outgrows it): one pass, no counting.  A row that overflows
+    // makes this and later assemblies of the plan count first and fill exact CSR rows.
+    bool tdone = false;
+    if (!P.tpad_off) {
+        dreserve(P, P.d_tadj, (size_t)c.nE * P.cap);
+        dreserve(P, P.d_tcut, (size_t)c.nE * P.cap);
+        k_tptr_padded<<<nblk(c.nE + 1, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tptr);
+        HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
+        HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
+        k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
+                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
+        k_tcheck<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tF, P.d_tcB, P.d_counters);
+        HIPCHECK(hipGetLastError());
+        read_counters(P, h);
+        if (h[CNT_OVERFLOW] & 16ull) {
+            P.tpad_off = true;
+            unsigned long long zero = 0;
+            HIPCHECK(hipMemcpyAsync(P.d_counters + CNT_OVERFLOW, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+        } else {
+            tdone = true;
+        }
+    }
+    if (!tdone) {
+        HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
+        k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr);
+        scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
+        long long ntadj = 0;
+        HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        dreserve(P, P.d_tadj, (size_t)ntadj);
+        dreserve(P, P.d_tcut, (size_t)ntadj);
+        HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
+        HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
+        k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
+                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
+    }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);
 
@@ -3233,7 +3283,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     PatArgs pa;
     pa.nNodes = P.nNodes; pa.is_DG = c.is_DG; pa.cap = P.cap; pa.tri = P.d_tri; pa.vptr = P.d_vptr;
     pa.vel = P.d_vel; pa.nbr = P.d_nbr; pa.cntF = P.d_cntF; pa.cntB = P.d_cntB; pa.tptr = P.d_tptr;
-    pa.tadj = P.d_tadj; pa.counters = P.d_counters;
+    pa.tadj = P.d_tadj; pa.tF = P.d_tF; pa.tB = P.d_tcB; pa.counters = P.d_counters;
     int*& d_rowcnt = P.d_wrowcnt;
     dreserve(P, d_rowcnt, (size_t)P.nNodes + 1);
     dreserve(P, P.d_rowptr, (size_t)P.nNodes + 1);
@@ -3333,7 +3383,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
             dreserve(P, P.d_P, (size_t)ptotal * P.oo);
             P.ptotal = ptotal * P.oo;
         }
-        na.tptr = P.d_tptr; na.tadj = P.d_tadj; na.poff = P.d_poff; na.P = P.d_P;
+        na.tptr = P.d_tptr; na.tadj = P.d_tadj; na.tF = P.d_tF; na.poff = P.d_poff; na.P = P.d_P;
         long long hn = next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
         for (int attempt = 0;; attempt++) {
             NumArgs ni = na;
@@ -3636,7 +3686,7 @@ void plan_free(Plan& P) {
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
-                    P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage};
+                    P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     void* wp[] = {P.d_wcnt, P.d_wrowcnt, P.d_wptmp, P.d_wrowOf, P.d_wplen, P.d_wocnt};
