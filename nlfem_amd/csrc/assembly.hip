@@ -1946,28 +1946,41 @@ struct ConstantItem {
 
 template <int OD>
 NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* iacc, double* pacc, double* tls) {
-    constexpr int OO = OD * OD, NB = 9 * OO;
+    constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     const int nPx = a.q.nPx;
-    const int t = threadIdx.x;
-    if (t < np) {
-        double tN[NB], tLP[NB];
-#pragma unroll
-        for (int o = 0; o < NB; o++) { tN[o] = 0; tLP[o] = 0; }
-        for (int k = 0; k < nPx; k++) {
-            const double* ia = iacc + ((size_t)t * nPx + k) * 10 * OO;
-            double psx[3] = {a.q.psix[3 * k], a.q.psix[3 * k + 1], a.q.psix[3 * k + 2]};
-            outer_point<OD, false>(psx, absA * a.q.dx[k], ia, ia + OO, nullptr, tN);
-            add_lp<OD>(ia + 4 * OO, a.q.dx[k] * absA, tLP);
+    const int nw = np * 2 * NB;
+    for (int t = threadIdx.x; t < nw + nPx * OO; t += 256) {
+        if (t < nw) {
+            // one entry of one pair: termNonloc[N ra + cb] (outer_point) or termLocalPrime (add_lp)
+            const int p = t / (2 * NB), e = t - p * 2 * NB;
+            const double* ia = iacc + (size_t)p * nPx * 10 * OO;
+            double v = 0;
+            if (e < NB) {
+                const int ra = e / N, cb = e - ra * N;
+                const int ai = ra / OD, src = OO + (ra % OD) * OD + OO * (cb / OD) + (cb % OD);
+                for (int k = 0; k < nPx; k++) {
+#pragma clang fp contract(fast)
+                    v += absA * a.q.dx[k] * a.q.psix[3 * k + ai] * ia[k * 10 * OO + src];
+                }
+            } else {
+                const int f = e - NB;  // a * 3 OO + (o / OD) * 3 OD + b * OD + o % OD
+                const int aa = f / (3 * OO), r1 = f - aa * 3 * OO;
+                const int oq = r1 / (3 * OD), r2 = r1 - oq * 3 * OD;
+                const int bb = r2 / OD, o = oq * OD + (r2 - bb * OD);
+                const int src = 4 * OO + sym6(aa, bb) * OO + o;
+                for (int k = 0; k < nPx; k++) {
+#pragma clang fp contract(fast)
+                    v += a.q.dx[k] * absA * ia[k * 10 * OO + src];
+                }
+            }
+            pacc[t] = v;
+        } else {
+            // innerLocal summed over the chunk's pairs per (outer point, component)
+            const int k = (t - nw) / OO, o = (t - nw) % OO;
+            double v = 0;
+            for (int p = 0; p < np; p++) v += iacc[((size_t)p * nPx + k) * 10 * OO + o];
+            tls[k * OO + o] += v;
         }
-        double* pa = pacc + t * 2 * NB;
-#pragma unroll
-        for (int o = 0; o < NB; o++) { pa[o] = tN[o]; pa[NB + o] = tLP[o]; }
-    } else if (t < np + nPx * OO) {
-        // innerLocal summed over the chunk's pairs per (outer point, component)
-        const int k = (t - np) / OO, o = (t - np) % OO;
-        double v = 0;
-        for (int p = 0; p < np; p++) v += iacc[((size_t)p * nPx + k) * 10 * OO + o];
-        tls[k * OO + o] += v;
     }
 }
 
@@ -1980,7 +1993,8 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     double2* pts = (double2*)ldsc;                                  // [CUT_ITEMS][CUT_MAXP]
     double* iacc = (double*)(pts + CUT_ITEMS * CUT_MAXP);          // [CUT_ITEMS][10*OO]: innerLocal, innerNonloc, sLP
     double* pacc = iacc + CUT_ITEMS * 10 * OO;                      // [CP][2*NB]: termNonloc, termLocalPrime
-    int* hk = (int*)(pacc + CUT_MAXPAIRS * 2 * NB);                 // [HN]
+    const int CP = min(CUT_ITEMS / a.q.nPx, CUT_MAXPAIRS);          // pairs per chunk
+    int* hk = (int*)(pacc + CP * 2 * NB);                           // [HN]
     int* hv = hk + a.HN;                                            // [HN][3]
     __shared__ int ntri[CUT_ITEMS];
     __shared__ int tpre[CUT_ITEMS + 1];
@@ -1997,7 +2011,6 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     const int nB = a.sur.list_in ? a.sur.rcnt[aT] : a.cntB[aT];
     if (nB == 0) return;
     const int nPx = a.q.nPx;
-    const int CP = min(CUT_ITEMS / nPx, CUT_MAXPAIRS);
     int rA[3];
     bool labA[3];
     for (int j = 0; j < 3; j++) {
@@ -3496,8 +3509,9 @@ void plan_assemble(Plan& P, const Estimates& est) {
             nc.sur.list_in = 1;
         }
         nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+        const int cpairs = std::min(CUT_ITEMS / std::max(c.q.nPx, 1), CUT_MAXPAIRS);
         const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP +
-                          sizeof(double) * (CUT_ITEMS * 10 * P.oo + CUT_MAXPAIRS * 18 * P.oo) + (size_t)nc.HN * 16;
+                          sizeof(double) * (CUT_ITEMS * 10 * P.oo + cpairs * 18 * P.oo) + (size_t)nc.HN * 16;
         if (lb > 160 * 1024) throw std::length_error("cut-pair LDS exceeds 160 KiB");
 #define LAUNCH_CUT(KER, OD)                                                                                          \
     do {                                                                                                             \
