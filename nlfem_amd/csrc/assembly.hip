@@ -1364,8 +1364,11 @@ template <int OD>
 NLFEM_DEV void cut_scatter(const NumArgs& a, int np, const int* pbs, const double* pacc, const int* rA,
                            const bool* labA, const int* hk, const int* hv) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
-    for (int t = threadIdx.x; t < np * 9; t += 256) {
-        const int p = t / 9, i = (t % 9) / 3, j = t % 3;
+    // one thread per (pair, block (i, j), component (ao, bo))
+    for (int t = threadIdx.x; t < np * 9 * OO; t += 256) {
+        const int p = t / (9 * OO), r = t - p * 9 * OO;
+        const int ij = r / OO, o = r - ij * OO;
+        const int i = ij / 3, j = ij % 3, ao = o / OD, bo = o % OD;
         const int b = pbs[p];
         if (b < 0) continue;
         const double* tN = pacc + p * 2 * NB;
@@ -1375,21 +1378,13 @@ NLFEM_DEV void cut_scatter(const NumArgs& a, int np, const int* pbs, const doubl
         const int s = lds_find(hk, a.HN, rBj);
         if (s < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); continue; }
         const long long pos = a.rowptr[rA[i]] + hv[3 * s + i];  // entry (aAdx[i], bAdx[j])
-#pragma unroll
-        for (int ao = 0; ao < OD; ao++)
-#pragma unroll
-            for (int bo = 0; bo < OD; bo++) {
-                const double v = tN[N * (i * OD + ao) + j * OD + bo];
-                if (labA[i]) atomicAdd(&a.val[pos * OO + ao * OD + bo], -v);
-                if (labBj) atomicAdd(&a.val[(long long)a.tpos[pos] * OO + bo * OD + ao], -v);
-            }
-        if (labBj) {  // +termLocalPrime[N (i OD+ao) + j OD+bo] into (bAdx[j], bAdx[i])
+        const double v = tN[N * (i * OD + ao) + j * OD + bo];
+        if (labA[i]) atomicAdd(&a.val[pos * OO + ao * OD + bo], -v);
+        if (labBj) {
+            atomicAdd(&a.val[(long long)a.tpos[pos] * OO + bo * OD + ao], -v);
+            // +termLocalPrime[N (i OD+ao) + j OD+bo] into (bAdx[j], bAdx[i])
             const long long q = a.posTT[9 * b + 3 * j + i];
-#pragma unroll
-            for (int ao = 0; ao < OD; ao++)
-#pragma unroll
-                for (int bo = 0; bo < OD; bo++)
-                    atomicAdd(&a.val[q * OO + bo * OD + ao], tLP[N * (i * OD + ao) + j * OD + bo]);
+            atomicAdd(&a.val[q * OO + bo * OD + ao], tLP[N * (i * OD + ao) + j * OD + bo]);
         }
     }
 }
