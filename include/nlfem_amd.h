@@ -107,6 +107,15 @@ int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
  * parts of different resolution, pass the values of the part with the most elements per
  * horizon.  Capacities still grow on overflow. */
 int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double amean);
+/* Force the general data structures that the plan otherwise uses only when the element
+ * numbering is not local or a fixed-size layout overflows (no counterpart in the reference;
+ * for checking those paths and for meshes whose numbering is known to be poor).  Bits:
+ * NLFEM_PATH_BFS_HASH: hashed BFS visited sets; NLFEM_PATH_PATTERN_HASH: hashed pattern
+ * element sets; NLFEM_PATH_TRANSPOSE_CSR: transposed relation counted into exact CSR rows. */
+#define NLFEM_PATH_BFS_HASH 1
+#define NLFEM_PATH_PATTERN_HASH 2
+#define NLFEM_PATH_TRANSPOSE_CSR 4
+int nlfem_plan_force_paths(nlfem_plan* plan, int flags);
 int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm, float tol,
                              double smin, double smax);
 /* Training samples from the exact FP64 cut-pair sums of an assembled plan: feat[maxn][12],

@@ -45,6 +45,7 @@ SIGNATURES = {
     "nlfem_load_vector": (C.c_int, [C.c_int, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _cp,
                                     C.c_int, _dp]),
     "nlfem_plan_set_estimates": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "nlfem_plan_force_paths": (C.c_int, [C.c_void_p, C.c_int]),
     "nlfem_plan_set_surrogate": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp, C.c_float, C.c_double, C.c_double]),
     "nlfem_plan_surrogate_samples": (C.c_int, [C.c_void_p, C.c_longlong, C.c_uint, C.c_uint, C.c_double, C.c_double,
                                                _fp, _fp, _llp]),

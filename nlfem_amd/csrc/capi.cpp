@@ -308,6 +308,15 @@ extern "C" int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double am
     return NLFEM_OK;
 }
 
+extern "C" int nlfem_plan_force_paths(nlfem_plan* plan, int flags) {
+    if (!plan) return fail(NLFEM_EINVAL, "null plan");
+    if (flags & ~7) return fail(NLFEM_EINVAL, "unknown path flags");
+    if (flags & NLFEM_PATH_BFS_HASH) plan->P.bfs_nobm = true;
+    if (flags & NLFEM_PATH_PATTERN_HASH) plan->P.pat_nobm = true;
+    if (flags & NLFEM_PATH_TRANSPOSE_CSR) plan->P.tpad_off = true;
+    return NLFEM_OK;
+}
+
 extern "C" int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm,
                                         float tol, double smin, double smax) {
     if (!plan) return fail(NLFEM_EINVAL, "null plan");

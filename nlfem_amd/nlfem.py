@@ -97,6 +97,15 @@ class Assembler:
         if sur:
             self.set_surrogate(**(sur if isinstance(sur, dict) else {}))
 
+    PATH_BFS_HASH, PATH_PATTERN_HASH, PATH_TRANSPOSE_CSR = 1, 2, 4
+
+    def force_paths(self, flags):
+        """Use the general data structures (hashed BFS visited sets, hashed pattern element
+        sets, counted CSR transposed rows) that are otherwise taken only for poorly numbered
+        meshes or after a fixed-size layout overflowed."""
+        check(lib().nlfem_plan_force_paths(self._h, int(flags)))
+        return self
+
     def set_surrogate(self, model="default", tol=None, smin=None, smax=None, enabled=True):
         """Route cut pairs through the surrogate networks of `model` (a path written by
         nlfem_amd.surrogate.save, or "default"); pairs outside the networks' domain or with an

@@ -22,8 +22,10 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 
 
-def _gpu(mesh, kernel, conf):
+def _gpu(mesh, kernel, conf, paths=0):
     asm = nlfem.Assembler(mesh, kernel, conf)
+    if paths:
+        asm.force_paths(paths)
     try:
         asm.assemble()
         A = asm.csr()
@@ -34,10 +36,10 @@ def _gpu(mesh, kernel, conf):
     return A, nb, st
 
 
-def _check_case(name, mesh, kernel, conf, tol=FP64_TOL):
-    print(f"[parity] case={name} nE={mesh['nE']} kernel={kernel['function']}", flush=True)
+def _check_case(name, mesh, kernel, conf, tol=FP64_TOL, paths=0):
+    print(f"[parity] case={name} nE={mesh['nE']} kernel={kernel['function']} paths={paths}", flush=True)
     ref = oracle.par_system(mesh, kernel, conf, keep_neighbours=True)
-    A, (ptr, nbr, cls), st = _gpu(mesh, kernel, conf)
+    A, (ptr, nbr, cls), st = _gpu(mesh, kernel, conf, paths)
     # neighbour lists: identical sets per outer element
     assert np.array_equal(ptr, ref["nbr_ptr"]), "neighbour list lengths differ"
     for g, r in zip(sorted_lists(ptr, nbr), sorted_lists(ref["nbr_ptr"], ref["nbr"])):
@@ -182,3 +184,17 @@ def test_parity_shuffled_element_numbering():
     perm = np.random.default_rng(3).permutation(m["nE"])
     ms = mesh_dict(m["elements"][perm], m["elementLabels"][perm], m["vertices"], m["vertexLabels"])
     _check_case("shuffled", ms, {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration())
+
+
+@pytest.mark.gpu
+def test_parity_general_paths():
+    """The hashed BFS visited sets, hashed pattern element sets and counted CSR transposed rows,
+    forced on a locally numbered mesh, give the same lists, pattern and values."""
+    from nlfem_amd.mesh import unit_square
+    from nlfem_amd.config import make_configuration
+    m = unit_square(40, 0.1, jitter=0.15, seed=5)
+    flags = nlfem.Assembler.PATH_BFS_HASH | nlfem.Assembler.PATH_PATTERN_HASH | nlfem.Assembler.PATH_TRANSPOSE_CSR
+    _check_case("general-paths", m, {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration(),
+                paths=flags)
+    _check_case("csr-transpose", m, {"function": "parabola", "horizon": 0.1, "outputdim": 1}, make_configuration(),
+                paths=nlfem.Assembler.PATH_TRANSPOSE_CSR)
