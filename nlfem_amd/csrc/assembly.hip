@@ -1360,9 +1360,9 @@ NLFEM_DEV void scatter_pair(const NumArgs& a, const int* rA, const bool* labA, i
 // into (bAdx[j], aAdx[i]); +termLocalPrime into (bAdx[j], bAdx[i]).  Positions of row aAdx[i]
 // come from the block's LDS column map (hk, hv), transposed entries through tpos, inner
 // element entries through posTT.  Pairs with pbs[p] < 0 are skipped.
-template <int OD>
+template <int OD, typename HV>
 NLFEM_DEV void cut_scatter(const NumArgs& a, int np, const int* pbs, const double* pacc, const int* rA,
-                           const bool* labA, const int* hk, const int* hv) {
+                           const bool* labA, const int* hk, const HV* hv) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     // one thread per (pair, block (i, j), component (ao, bo))
     for (int t = threadIdx.x; t < np * 9 * OO; t += 256) {
@@ -1797,6 +1797,13 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
 constexpr int CUT_ITEMS = 128;
 constexpr int CUT_MAXP = 9;
 constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= CUT_ITEMS / CUT_MAXPAIRS = 4
+// Dynamic LDS of k_cut: polygons (shared with the pair sums), per-item sums, the outer rule's
+// tables, the chunk's LU factors, and the column map (int keys, 16-bit row positions).
+static size_t cut_lds_bytes(int nPx, int oo, int HN) {
+    const int cp = std::min(CUT_ITEMS / std::max(nPx, 1), CUT_MAXPAIRS);
+    return 16 * (size_t)CUT_ITEMS * CUT_MAXP + 8 * ((size_t)CUT_ITEMS * 10 * oo + (size_t)nPx * (oo + 6)) +
+           sizeof(PivotLU) * (size_t)cp + 10 * (size_t)HN;
+}
 
 // Surrogate item (outer point x, inner element E): ZERO when B(x, delta) misses E, FULL when
 // it contains E, PARTIAL otherwise; OUT when the item lies outside the network's domain
@@ -1939,8 +1946,10 @@ struct ConstantItem {
     }
 };
 
+// wa[3 k + i] = |aT| dx_k psi_i(x_k) and wd[k] = dx_k |aT| are the block's outer weights in LDS.
 template <int OD>
-NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* iacc, double* pacc, double* tls) {
+NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const double* wd, const double* iacc,
+                         double* pacc, double* tls) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     const int nPx = a.q.nPx;
     const int nw = np * 2 * NB;
@@ -1953,9 +1962,10 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* ia
             if (e < NB) {
                 const int ra = e / N, cb = e - ra * N;
                 const int ai = ra / OD, src = OO + (ra % OD) * OD + OO * (cb / OD) + (cb % OD);
+#pragma unroll 4
                 for (int k = 0; k < nPx; k++) {
 #pragma clang fp contract(fast)
-                    v += absA * a.q.dx[k] * a.q.psix[3 * k + ai] * ia[k * 10 * OO + src];
+                    v += wa[3 * k + ai] * ia[k * 10 * OO + src];
                 }
             } else {
                 const int f = e - NB;  // a * 3 OO + (o / OD) * 3 OD + b * OD + o % OD
@@ -1963,9 +1973,10 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* ia
                 const int oq = r1 / (3 * OD), r2 = r1 - oq * 3 * OD;
                 const int bb = r2 / OD, o = oq * OD + (r2 - bb * OD);
                 const int src = 4 * OO + sym6(aa, bb) * OO + o;
+#pragma unroll 4
                 for (int k = 0; k < nPx; k++) {
 #pragma clang fp contract(fast)
-                    v += a.q.dx[k] * absA * ia[k * 10 * OO + src];
+                    v += wd[k] * ia[k * 10 * OO + src];
                 }
             }
             pacc[t] = v;
@@ -1984,18 +1995,25 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, double absA, const double* ia
 template <int KER, int OD, typename T, int MODE = 0>
 __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    // dynamic LDS, cut_lds_bytes() on the host: the polygons of phases A-B share their space
+    // with the pair sums of phases C-D (CP * 2 NB <= 2 * CUT_ITEMS * CUT_MAXP doubles)
+    const int nPx_ = a.q.nPx;
+    const int CP = min(CUT_ITEMS / nPx_, CUT_MAXPAIRS);             // pairs per chunk
     extern __shared__ double ldsc[];
     double2* pts = (double2*)ldsc;                                  // [CUT_ITEMS][CUT_MAXP]
+    double* pacc = ldsc;                                            // [CP][2*NB]: termNonloc, termLocalPrime
     double* iacc = (double*)(pts + CUT_ITEMS * CUT_MAXP);          // [CUT_ITEMS][10*OO]: innerLocal, innerNonloc, sLP
-    double* pacc = iacc + CUT_ITEMS * 10 * OO;                      // [CP][2*NB]: termNonloc, termLocalPrime
-    const int CP = min(CUT_ITEMS / a.q.nPx, CUT_MAXPAIRS);          // pairs per chunk
-    int* hk = (int*)(pacc + CP * 2 * NB);                           // [HN]
-    int* hv = hk + a.HN;                                            // [HN][3]
+    double* tls = iacc + CUT_ITEMS * 10 * OO;                       // [nPx][OO]
+    double* xk = tls + nPx_ * OO;                                   // [nPx][2] outer points of aT
+    double* wa = xk + 2 * nPx_;                                     // [nPx][3] |aT| dx_k psi_i(x_k)
+    double* wd = wa + 3 * nPx_;                                     // [nPx] dx_k |aT|
+    PivotLU* lus = (PivotLU*)(wd + nPx_);                           // [CP]
+    int* hk = (int*)(lus + CP);                                     // [HN]
+    unsigned short* hv = (unsigned short*)(hk + a.HN);              // [HN][3] positions in aT's rows
     __shared__ int ntri[CUT_ITEMS];
     __shared__ int tpre[CUT_ITEMS + 1];
     __shared__ int pb_[CUT_MAXPAIRS];                               // inner element of each chunk pair
-    __shared__ PivotLU lus[CUT_MAXPAIRS];
-    __shared__ double tls[MAXQ * OO];
+    __shared__ unsigned char tmap[CUT_ITEMS * (CUT_MAXP - 2)];      // fan triangle -> item
     __shared__ unsigned subtri;
     __shared__ int wsum[2];
     const int aT = blockIdx.x;
@@ -2013,7 +2031,13 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
     }
     for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
-    for (int i = tid; i < MAXQ * OO; i += 256) tls[i] = 0.0;
+    for (int i = tid; i < nPx * OO; i += 256) tls[i] = 0.0;
+    if (tid < nPx) {
+        toPhys(A.E, a.q.Px[2 * tid], a.q.Px[2 * tid + 1], &xk[2 * tid]);
+        const double c1 = A.absDet * a.q.dx[tid];
+        for (int i = 0; i < 3; i++) wa[3 * tid + i] = c1 * a.q.psix[3 * tid + i];
+        wd[tid] = a.q.dx[tid] * A.absDet;
+    }
     if (tid == 0) subtri = 0;
     __syncthreads();
     for (int j = 0; j < 3; j++) {
@@ -2021,7 +2045,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         const int len = (int)(a.rowptr[rA[j] + 1] - r0);
         for (int i = tid; i < len; i += 256) {
             int s = lds_slot(hk, a.HN, a.col[r0 + i]);
-            if (s >= 0) hv[3 * s + j] = i;
+            if (s >= 0) hv[3 * s + j] = (unsigned short)i;
             else atomicOr(&a.counters[CNT_ERROR], 16ull);
         }
     }
@@ -2044,8 +2068,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         if (tid < nit) {
             const int p = tid / nPx, k = tid % nPx;
             const ElemRec B = a.elem[pb_[p]];
-            double x[2];
-            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+            const double x[2] = {xk[2 * k], xk[2 * k + 1]};
             int nt = 0;
             if (!tri_far(x[0], x[1], B.E, a.sq_far)) {
                 if constexpr (KER == KER_CONSTANT && OD == 1) {  // integrated here: no fan triangles for phase B
@@ -2079,27 +2102,22 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             if (lane == 63 && w < 2) wsum[w] = v;
             __syncthreads();
             if (w == 1) v += wsum[0];
-            if (tid < CUT_ITEMS) tpre[tid + 1] = v;
+            if (tid < CUT_ITEMS) {
+                tpre[tid + 1] = v;
+                for (int q = v - ntri[tid]; q < v; q++) tmap[q] = (unsigned char)tid;
+            }
             if (tid == 0) tpre[0] = 0;
         }
         __syncthreads();
         // B: fan triangles
         const int ntot = tpre[nit];
         for (int t = tid; t < ntot; t += 256) {
-            int lo = 0, hi = nit - 1;  // item with tpre[it] <= t < tpre[it+1]
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (tpre[mid] <= t) lo = mid;
-                else hi = mid - 1;
-            }
-            const int it = lo, j = t - tpre[it];
+            const int it = tmap[t], j = t - tpre[it];
             const int p = it / nPx, k = it % nPx;
             const double2 R0 = pts[it * CUT_MAXP], R1 = pts[it * CUT_MAXP + j + 1], R2 = pts[it * CUT_MAXP + j + 2];
             const double rT[6] = {R0.x, R0.y, R1.x, R1.y, R2.x, R2.y};
             const double rTdet = absDet(rT);
-            double x[2];
-            toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
-            const double wk = a.q.dx[k] * absA;
+            const double x[2] = {xk[2 * k], xk[2 * k + 1]};
             const PivotLU lu = lus[p];
             double inL[OO], inN[3 * OO], sLP[6 * OO];
 #pragma unroll
@@ -2154,7 +2172,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         }
         // C: outer accumulation (src/integration.cpp:508-537), one thread per (pair, entry) of
         // termNonloc, termLocalPrime and termLocal; termLocal is summed over the block in LDS
-        cut_outer<OD>(a, np, absA, iacc, pacc, tls);
+        cut_outer<OD>(a, np, wa, wd, iacc, pacc, tls);
         __syncthreads();
         // D: scatter (src/Cassemble.cpp:721-770)
         cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
@@ -3504,10 +3522,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
             nc.sur.list_in = 1;
         }
         nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
-        const int cpairs = std::min(CUT_ITEMS / std::max(c.q.nPx, 1), CUT_MAXPAIRS);
-        const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP +
-                          sizeof(double) * (CUT_ITEMS * 10 * P.oo + cpairs * 18 * P.oo) + (size_t)nc.HN * 16;
-        if (lb > 160 * 1024) throw std::length_error("cut-pair LDS exceeds 160 KiB");
+        const size_t lb = cut_lds_bytes(c.q.nPx, P.oo, nc.HN);
+        if (lb > 160 * 1024 || P.maxRow > 65535) throw std::length_error("cut-pair LDS exceeds 160 KiB");
 #define LAUNCH_CUT(KER, OD)                                                                                          \
     do {                                                                                                             \
         if (c.precision == 1) {                                                                                      \
@@ -3663,8 +3679,8 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
     set_surrogate_args(P, nc.sur);
     nc.sur.sfeat = dF; nc.sur.starg = dT; nc.sur.smaxn = maxn; nc.sur.stride = stride; nc.sur.salt = salt;
     nc.sur.smin = smin; nc.sur.smax = smax;
-    const size_t lb = sizeof(double2) * CUT_ITEMS * CUT_MAXP + sizeof(double) * (CUT_ITEMS * 10 + CUT_MAXPAIRS * 18) +
-                      (size_t)nc.HN * 16;
+    const size_t lb = cut_lds_bytes(c.q.nPx, 1, nc.HN);
+    if (lb > 160 * 1024 || P.maxRow > 65535) throw std::length_error("cut-pair LDS exceeds 160 KiB");
     unsigned long long h[16];
     try {
         HIPCHECK(hipMemsetAsync(P.d_counters + CNT_SAMPLES, 0, sizeof(unsigned long long), st));
