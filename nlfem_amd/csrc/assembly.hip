@@ -1091,12 +1091,37 @@ struct NumArgs {
     double* stage;
 };
 
-// Kernel value(s) at z = x - y: OO entries (row-major od x od).
+// 1 / sqrt(r2) for r2 > 0: the hardware estimate (v_rsq_f64, relative error below 2^-22)
+// refined by two Newton-Raphson steps y <- y + y (1 - r2 y^2) / 2, which leave an error of a
+// few units in the last place (the reference's 1 / sqrt and 1 / pow(sqrt, 3) round once or
+// twice; the difference is far below the FP64 tolerance of the assembled entries).
+NLFEM_DEV double rsqrt_nr(double r2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(r2);
+#else
+    double y = 1.0 / std::sqrt(r2);
+#endif
+    for (int it = 0; it < 2; it++) {
+        const double e = fma(-(r2 * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// Kernel value(s) at z = x - y: OO entries (row-major od x od).  The microelastic kernels'
+// 1 / |z| and 1 / |z|^3 come from rsqrt_nr in FP64.
 template <int KER, int OO, typename T>
 NLFEM_DEV void kernel_eval(T z0, T z1, const KernelParams& kp, T* K) {
+    constexpr bool F64 = sizeof(T) == sizeof(double);
     if constexpr (KER == KER_MICRO_FIELD) {
-        T r = sqrt(z0 * z0 + z1 * z1);
-        T cd = (T)kp.c * ((T)1.0 / (r * r * r));
+        T cd;
+        if constexpr (F64) {
+            const double ri = rsqrt_nr(z0 * z0 + z1 * z1);
+            cd = kp.c * (ri * ri * ri);
+        } else {
+            T r = sqrt(z0 * z0 + z1 * z1);
+            cd = (T)kp.c * ((T)1.0 / (r * r * r));
+        }
         K[0] = cd * z0 * z0;
         K[1] = cd * z0 * z1;
         K[2] = cd * z1 * z0;
@@ -1105,9 +1130,29 @@ NLFEM_DEV void kernel_eval(T z0, T z1, const KernelParams& kp, T* K) {
         if constexpr (KER == KER_CONSTANT) K[0] = (T)kp.c;
         else if constexpr (KER == KER_PARABOLA) K[0] = (T)kp.c * ((T)kp.sqdelta - (z0 * z0 + z1 * z1));
         else if constexpr (KER == KER_FRACTIONAL) K[0] = (T)kp.c * ((T)1.0 / pow(sqrt(z0 * z0 + z1 * z1), (T)kp.exponent));
+        else if constexpr (F64) K[0] = kp.c * rsqrt_nr(z0 * z0 + z1 * z1);
         else K[0] = (T)kp.c * ((T)1.0 / sqrt(z0 * z0 + z1 * z1));
     }
 }
+
+// The basis of bT on a fan triangle rT = (R0, R1, R2).  toRef is affine, so its values at
+// the inner rule's points P follow from its values at the corners (the reference's pivoted
+// toRef, src/mathhelpers.cpp:227-240, at each corner):
+// p(P) = p(R0) + P0 (p(R1) - p(R0)) + P1 (p(R2) - p(R0)), which is what toRef(toPhys(rT, P))
+// gives up to rounding (a few units in the last place of psi in [0, 1]).
+struct FanBasis {
+    double a0, b0, c0, a1, b1, c1;
+    NLFEM_DEV FanBasis(const PivotLU& lu, const double* rT) {
+        double q0[3], q1[3];
+        for (int c = 0; c < 3; c++) lu.solve(rT[2 * c], rT[2 * c + 1], q0[c], q1[c]);
+        a0 = q0[0]; b0 = q0[1] - q0[0]; c0 = q0[2] - q0[0];
+        a1 = q1[0]; b1 = q1[1] - q1[0]; c1 = q1[2] - q1[0];
+    }
+    NLFEM_DEV void at(double P0, double P1, double& p0, double& p1) const {
+        p0 = fma(P1, c0, fma(P0, b0, a0));
+        p1 = fma(P1, c1, fma(P0, b1, a1));
+    }
+};
 
 // Accumulators of one pair.  tN is termNonloc; for symmetric kernels termNonlocPrime equals
 // termNonloc bit for bit (k(y,x) is computed from the same |z| / z z^T products), so it is not
@@ -1237,6 +1282,7 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
             const double rT[6] = {r0, r1, s0, s1, t0, t1};
             const double rTdet = absDet(rT);
             ntri++;
+            const FanBasis fb(lu, rT);
             for (int i = 0; i < a.q.nPy; i++) {
                 double y[2];
                 toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
@@ -1244,7 +1290,7 @@ NLFEM_DEV void pair_cut(const NumArgs& a, const ElemRec& A, const ElemRec& B, do
                 kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
                 double p0, p1;
                 if constexpr (sizeof(T) == sizeof(double)) {
-                    lu.solve(y[0], y[1], p0, p1);  // the reference's pivoted toRef, bit for bit
+                    fb.at(a.q.Py[2 * i], a.q.Py[2 * i + 1], p0, p1);
                 } else {
                     const double d0 = y[0] - im.e0, d1 = y[1] - im.e1;
                     p0 = im.i00 * d0 + im.i01 * d1;
@@ -2118,7 +2164,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
             const double rT[6] = {R0.x, R0.y, R1.x, R1.y, R2.x, R2.y};
             const double rTdet = absDet(rT);
             const double x[2] = {xk[2 * k], xk[2 * k + 1]};
-            const PivotLU lu = lus[p];
+            const FanBasis fb(lus[p], rT);
             double inL[OO], inN[3 * OO], sLP[6 * OO];
 #pragma unroll
             for (int o = 0; o < OO; o++) inL[o] = 0;
@@ -2132,7 +2178,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
                 T K[OO];
                 kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
                 double p0, p1;
-                lu.solve(y[0], y[1], p0, p1);
+                fb.at(a.q.Py[2 * i], a.q.Py[2 * i + 1], p0, p1);
                 T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
                 inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, inL, inN, sLP);
             }
@@ -2284,13 +2330,14 @@ __device__ __forceinline__ void cut8_body(const NumArgs& a) {
                         const double rT[6] = {r0, r1, s0, s1, t0, t1};
                         const double rTdet = absDet(rT);
                         mytri++;
+                        const FanBasis fb(lu, rT);
                         for (int i = 0; i < a.q.nPy; i++) {
                             double y[2];
                             toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
                             T K[1];
                             kernel_eval<KER, 1, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
                             double p0, p1;
-                            lu.solve(y[0], y[1], p0, p1);
+                            fb.at(a.q.Py[2 * i], a.q.Py[2 * i + 1], p0, p1);
                             T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
                             inner_point<1, T>(K, (T)(a.q.dy[i] * rTdet), psi, &inL, inN, sLP);
                         }
