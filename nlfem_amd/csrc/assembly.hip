@@ -1195,7 +1195,11 @@ NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double* inL, double* i
     }
 }
 
-NLFEM_DEV constexpr int sym6(int a, int b) { return a <= b ? (a == 0 ? b : (a == 1 ? 2 + b : 5)) : sym6(b, a); }
+// Index of the symmetric pair (a, b) among (0,0) (0,1) (0,2) (1,1) (1,2) (2,2).
+NLFEM_DEV constexpr int sym6(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
 
 // termLocalPrime[a*OO*3 + (o/OD)*OD*3 + b*OD + o%OD] += psi_a psi_b k'[o] dy rTdet dx_k |aT|
 // (src/integration.cpp:462-476), with the inner sum taken first.
@@ -1622,7 +1626,7 @@ NLFEM_DEV ColMap colmap_build(int* h, int H, const int* col, const long long* rp
     return cm;
 }
 
-template <int KER, int OD, typename T>
+template <int KER, int OD, typename T, int NPY = 0>
 __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     extern __shared__ double ldsd[];
@@ -1649,6 +1653,181 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         rlen[j] = (int)(a.rowptr[rA[j] + 1] - rp[j]);
     }
     const ColMap cm = colmap_build(hk, a.HN, a.col, rp, rlen, labA, scan_tmp, a.counters);
+    // NPY > 0: the general (non-factorised) kernels with an inner rule of NPY points
+    constexpr bool FIXED = NPY > 0 && !(KER == KER_CONSTANT && OD == 1);
+    const int nPx = a.q.nPx;
+    const int lane = tid & 63, w = tid >> 6;
+    if constexpr (FIXED) {
+        // Inner rule of NPY points unrolled.  Per block: the outer points xo and inner points yi
+        // of T, and the outer weights wxa[k][i] = |T| dx_k psi_i(x_k), wl[k][sym(i,j)] =
+        // |T| dx_k psi_i(x_k) psi_j(x_k) that outer_point applies.
+        __shared__ double xo[2 * MAXQ], yi[2 * NPY], wxa[3 * MAXQ], wl[6 * MAXQ];
+        if (tid < nPx) {
+            toPhys(A.E, a.q.Px[2 * tid], a.q.Px[2 * tid + 1], &xo[2 * tid]);
+            const double ak = A.absDet * a.q.dx[tid];
+            for (int i = 0; i < 3; i++) {
+                wxa[3 * tid + i] = ak * a.q.psix[3 * tid + i];
+                for (int j = i; j < 3; j++) wl[6 * tid + sym6(i, j)] = ak * a.q.psix[3 * tid + i] * a.q.psix[3 * tid + j];
+            }
+        }
+        if (tid < NPY) toPhys(A.E, a.q.Py[2 * tid], a.q.Py[2 * tid + 1], &yi[2 * tid]);
+        __syncthreads();
+        // T as outer element (pair_interior with termLocal, without termLocalPrime).  termLocal
+        // depends on (i, j) through psi_i psi_j only: six symmetric sums per component.
+        double tL6[6 * OO];
+#pragma unroll
+        for (int t = 0; t < 6 * OO; t++) tL6[t] = 0;
+        const int nF = a.cntF[T0];
+        for (int p = tid; p < nF; p += 256) {
+            const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
+            if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const ElemRec B = a.elem[U];
+            double yu[2 * NPY];
+#pragma unroll
+            for (int l = 0; l < NPY; l++) toPhys(B.E, a.q.Py[2 * l], a.q.Py[2 * l + 1], &yu[2 * l]);
+            double tN[NB];
+#pragma unroll
+            for (int t = 0; t < NB; t++) tN[t] = 0;
+            for (int k = 0; k < nPx; k++) {
+                const double x0 = xo[2 * k], x1 = xo[2 * k + 1];
+                double inL[OO], inN[3 * OO];
+#pragma unroll
+                for (int o = 0; o < OO; o++) inL[o] = 0;
+#pragma unroll
+                for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+#pragma unroll
+                for (int l = 0; l < NPY; l++) {
+                    T K[OO];
+                    kernel_eval<KER, OO, T>((T)(x0 - yu[2 * l]), (T)(x1 - yu[2 * l + 1]), a.kp, K);
+                    const T psi[3] = {(T)a.q.psiy[3 * l], (T)a.q.psiy[3 * l + 1], (T)a.q.psiy[3 * l + 2]};
+                    inner_point<OD, T, true, true, false>(K, (T)(a.q.dy[l] * B.absDet), psi, inL, inN, nullptr);
+                }
+                {
+#pragma clang fp contract(fast)
+#pragma unroll
+                    for (int q = 0; q < 6; q++)
+#pragma unroll
+                        for (int o = 0; o < OO; o++) tL6[q * OO + o] += wl[6 * k + q] * inL[o];
+#pragma unroll
+                    for (int r = 0; r < N; r++)
+#pragma unroll
+                        for (int c = 0; c < N; c++)
+                            tN[N * r + c] += wxa[3 * k + r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int s = cm.find(node_of(a.tri, U, j, a.is_DG));
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if (!labA[i]) continue;
+                    if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+#pragma unroll
+                    for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                        for (int bo = 0; bo < OD; bo++)
+                            atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], -tN[N * (i * OD + ao) + j * OD + bo]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 6 * OO; t++) {
+            double v = tL6[t];
+            for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+            tL6[t] = v;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < N; r++)
+#pragma unroll
+                for (int c = 0; c < N; c++) red[w][N * r + c] = tL6[sym6(r / OD, c / OD) * OO + OD * (r % OD) + (c % OD)];
+        }
+        // T as inner element: pairs (U, T).  termLocalPrime of T sums, over the pairs and the
+        // outer points x_k of U, dx_k |U| psi_a psi_b(y_l) dy_l |T| k(x_k, y_l); the factor
+        // psi_a psi_b(y_l) dy_l |T| depends on l only, so M[l] = sum dx_k |U| k(x_k, y_l) is
+        // accumulated and the six products are applied once.
+        double M[NPY * OO];
+#pragma unroll
+        for (int t = 0; t < NPY * OO; t++) M[t] = 0;
+        const long long tb = a.tptr[T0], te = tb + a.tF[T0];
+        for (long long q = tb + tid; q < te; q += 256) {
+            const uint32_t e = (uint32_t)a.tadj[q];
+            if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const ElemRec B = a.elem[U];
+            double tN[NB];
+#pragma unroll
+            for (int t = 0; t < NB; t++) tN[t] = 0;
+            for (int k = 0; k < nPx; k++) {
+                double x[2];
+                toPhys(B.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+                const double wk = a.q.dx[k] * B.absDet;
+                double inN[3 * OO];
+#pragma unroll
+                for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+#pragma unroll
+                for (int l = 0; l < NPY; l++) {
+                    T K[OO];
+                    kernel_eval<KER, OO, T>((T)(x[0] - yi[2 * l]), (T)(x[1] - yi[2 * l + 1]), a.kp, K);
+                    const T psi[3] = {(T)a.q.psiy[3 * l], (T)a.q.psiy[3 * l + 1], (T)a.q.psiy[3 * l + 2]};
+                    inner_point<OD, T, false, true, false>(K, (T)(a.q.dy[l] * A.absDet), psi, nullptr, inN, nullptr);
+                    {
+#pragma clang fp contract(fast)
+#pragma unroll
+                        for (int o = 0; o < OO; o++) M[l * OO + o] += wk * (double)K[o];
+                    }
+                }
+                {
+#pragma clang fp contract(fast)
+                    const double ak = B.absDet * a.q.dx[k];
+                    double wi[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) wi[i] = ak * a.q.psix[3 * k + i];
+#pragma unroll
+                    for (int r = 0; r < N; r++)
+#pragma unroll
+                        for (int c = 0; c < N; c++)
+                            tN[N * r + c] += wi[r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i)
+                const int s = cm.find(node_of(a.tri, U, i, a.is_DG));
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    if (!labA[j]) continue;
+                    if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+#pragma unroll
+                    for (int ao = 0; ao < OD; ao++)
+#pragma unroll
+                        for (int bo = 0; bo < OD; bo++)
+                            atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -tN[N * (i * OD + ao) + j * OD + bo]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NPY * OO; t++) {
+            double v = M[t];
+            for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
+            M[t] = v;
+        }
+        if (lane == 0) {
+            // termLocalPrime[a * 3 OO + (o / OD) * 3 OD + b * OD + o % OD], as add_lp lays it out
+#pragma unroll
+            for (int ia = 0; ia < 3; ia++)
+#pragma unroll
+                for (int ib = 0; ib < 3; ib++)
+#pragma unroll
+                    for (int o = 0; o < OO; o++) {
+                        double v = 0;
+#pragma unroll
+                        for (int l = 0; l < NPY; l++)
+                            v += a.q.psiy[3 * l + ia] * a.q.psiy[3 * l + ib] * (a.q.dy[l] * A.absDet) * M[l * OO + o];
+                        red[w][NB + ia * OO * 3 + (o / OD) * OD * 3 + ib * OD + o % OD] = v;
+                    }
+        }
+    } else {
     double tL[NB], tLP[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) { tL[i] = 0; tLP[i] = 0; }
@@ -1745,12 +1924,12 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
             }
     }
     // block reduction of termLocal and termLocalPrime
-    const int lane = tid & 63, w = tid >> 6;
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         double v = tL[i], u = tLP[i];
         for (int m = 32; m > 0; m >>= 1) { v += shfl_xor_d(v, m); u += shfl_xor_d(u, m); }
         if (lane == 0) { red[w][i] = v; red[w][NB + i] = u; }
+    }
     }
     __syncthreads();
     for (int t = tid; t < 2 * NB; t += 256) {
@@ -3463,15 +3642,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
             ni.HN = (int)hn;
             const size_t lb = (size_t)hn * (24 * P.oo + 4);
             if (lb > 160 * 1024) throw std::length_error("interior LDS table exceeds 160 KiB");
-#define LAUNCH_INT(KER, OD)                                                                                         \
+#define LAUNCH_INT_N(KER, OD, NPY)                                                                                  \
     do {                                                                                                            \
         if (c.precision == 1) {                                                                                     \
-            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
-            k_interior<KER, OD, float><<<c.nE, 256, lb, st>>>(ni);                                                  \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, float, NPY>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_interior<KER, OD, float, NPY><<<c.nE, 256, lb, st>>>(ni);                                             \
         } else {                                                                                                    \
-            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
-            k_interior<KER, OD, double><<<c.nE, 256, lb, st>>>(ni);                                                 \
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_interior<KER, OD, double, NPY>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+            k_interior<KER, OD, double, NPY><<<c.nE, 256, lb, st>>>(ni);                                            \
         }                                                                                                           \
+    } while (0)
+// the general kernels with the 7-point inner rule: the rule unrolled
+#define LAUNCH_INT(KER, OD)                                                                                         \
+    do {                                                                                                            \
+        if (KER != KER_CONSTANT && c.q.nPy == 7) LAUNCH_INT_N(KER, OD, 7);                                          \
+        else LAUNCH_INT_N(KER, OD, 0);                                                                              \
     } while (0)
             switch (c.kernel) {
                 case KER_CONSTANT: LAUNCH_INT(KER_CONSTANT, 1); break;
@@ -3481,6 +3666,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
                 default: LAUNCH_INT(KER_MICRO_FIELD, 2); break;
             }
 #undef LAUNCH_INT
+#undef LAUNCH_INT_N
             HIPCHECK(hipGetLastError());
             read_counters(P, h);
             if (!(h[CNT_ERROR] & ERR_HASH)) break;
