@@ -98,6 +98,42 @@ def cpu_baseline(mesh, kernel, conf, budget_s=15.0):
                       f"{r['seconds']:.2f} s incl. dual graph and merge, OMP threads={cores}"}
 
 
+def oracle_matrix(mesh, kernel, conf):
+    """The FP64 CPU oracle's matrix of the whole mesh (all host cores) and its wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.perf_counter()
+    r = oracle.par_system(mesh, kernel, conf)
+    return r["A"], time.perf_counter() - t0
+
+
+def error_vs(B, A):
+    """Error of the GPU matrix B against the oracle matrix A: relative Frobenius norm, largest
+    entry difference relative to max|A|, whether the sparsity patterns are identical, and when
+    they are not, how many entries are in one pattern only and the largest of their values
+    relative to max|A| (exact zeros are dropped, so an entry that is rounding residue in one
+    matrix and exactly zero in the other appears in one pattern only)."""
+    import scipy.sparse.linalg as spla
+    A = A.tocsr()
+    B = B.tocsr()
+    A.sort_indices()
+    B.sort_indices()
+    D = (B - A).tocsr()
+    amax = float(abs(A).max()) if A.nnz else 0.0
+    dmax = float(abs(D).max()) if D.nnz else 0.0
+    out = {"rel_err_fro": float(spla.norm(D) / spla.norm(A)), "max_abs_err_rel": dmax / amax if amax else dmax,
+           "pattern_equal": bool(np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices))}
+    if not out["pattern_equal"]:
+        PA = A.copy(); PA.data[:] = 1.0
+        PB = B.copy(); PB.data[:] = 1.0
+        X = (PA - PB).tocoo()
+        X.eliminate_zeros()
+        vals = [abs(A[r, c]) + abs(B[r, c]) for r, c in zip(X.row[:1000], X.col[:1000])]
+        out["pattern_diff_entries"] = int(X.nnz)
+        out["pattern_diff_max_rel"] = float(max(vals) / amax) if vals and amax else 0.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +144,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the mixed-precision and surrogate+mixed timings of the same workload")
+                    help="skip the mixed-precision and surrogate+mixed timings of the same workload "
+                         "(batch: skip the per-mesh error checks)")
+    ap.add_argument("--vs-oracle", action="store_true",
+                    help="also assemble the whole workload with the FP64 CPU oracle (all host cores) and report "
+                         "each GPU matrix's error against it (batch: the meshes of at most 2,048 elements)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -170,11 +210,18 @@ def main():
         "phases_ms": {k[3:]: round(v, 3) for k, v in ms_phase.items()},
         "roofline": roof,
     }
+    A64 = asm.csr()
+    Aor = None
+    if args.vs_oracle:
+        print("[bench] oracle assembly of the whole mesh", file=sys.stderr, flush=True)
+        Aor, sec = oracle_matrix(mesh, kernel, conf)
+        out["vs_oracle"] = {"oracle_seconds": round(sec, 2), "oracle_threads": os.cpu_count() or 1,
+                            args.precision: error_vs(A64, Aor)}
+        print(f"[bench] oracle done in {sec:.1f} s", file=sys.stderr, flush=True)
     if not args.no_variants and args.precision == "fp64":
         # the same workload in mixed precision and with the cut-pair surrogate (constant kernel),
         # each compared with the FP64 matrix just assembled
         import scipy.sparse.linalg as spla
-        A64 = asm.csr()
         nrm = spla.norm(A64)
         variants = {"mixed": {"precision": "mixed"}}
         if kernel["function"] == "constant":
@@ -198,6 +245,8 @@ def main():
                  "rel_err_fro_vs_fp64": float(spla.norm(B - A64) / nrm),
                  "pattern_equal": bool(np.array_equal(B.indptr, A64.indptr) and np.array_equal(B.indices, A64.indices)),
                  "ms_cut": round(st2["ms_cut"], 3), "ms_interior": round(st2["ms_interior"], 3)}
+            if Aor is not None:
+                out["vs_oracle"][name] = error_vs(B, Aor)
             if "surrogate" in extra:
                 v["cut_pairs_by_surrogate"] = int(st2["sur_pairs"])
                 v["cut_pairs_exact_fallback"] = int(st2["sur_rejected"])
@@ -225,6 +274,8 @@ def run_batch(args):
     print(f"[bench] batch of {len(meshes)} meshes, {sum(m['nE'] for m in meshes)} elements", file=sys.stderr,
           flush=True)
     b = batch.BatchAssembler(meshes, deltas, kernel, conf)
+    cut_items, tune_ms = b.autotune()
+    print(f"[bench] autotune: {cut_items} items per chunk, cut ms per candidate {tune_ms}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         b.assemble()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
@@ -235,14 +286,22 @@ def run_batch(args):
     st = b.stats()
     blocks = b.matrices()
     b.close()
-    errs = []
+    errs, oerrs = [], []
     for (m, d), B in zip(zip(meshes, deltas), blocks):
+        if args.no_variants:  # timing runs (profiles) skip the per-mesh checks
+            break
         k = dict(kernel)
         k["horizon"] = d
         a = nlfem.Assembler(m, k, conf)
         A = a.assemble().csr()
         a.close()
         errs.append(float(spla.norm(A - B) / spla.norm(A)))
+        if args.vs_oracle and m["nE"] <= 2048:
+            Aor, sec = oracle_matrix(m, k, conf)
+            e = error_vs(B, Aor)
+            e.update({"nE": int(m["nE"]), "delta": round(float(d), 4), "oracle_seconds": round(sec, 2)})
+            oerrs.append(e)
+            print(f"[bench] oracle mesh nE={m['nE']} delta={d:.3f}: {e}", file=sys.stderr, flush=True)
     print(f"[bench] per-mesh errors done", file=sys.stderr, flush=True)
     out = {
         "metric": "element-pairs/s (assembled ordered element pairs per second, 48 meshes in one plan)",
@@ -251,11 +310,18 @@ def run_batch(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": BATCH, "meshes": len(meshes), "nE_total": int(sum(m["nE"] for m in meshes)),
                    "h": sorted({n for _, _, n in items}), "delta_range": [min(deltas), max(deltas)],
-                   "kernel": kernel["function"], "outdim": 2, "quadrature": "16/7"},
+                   "kernel": kernel["function"], "outdim": 2, "quadrature": "16/7",
+                   "cut_items_per_chunk": cut_items, "autotune_cut_ms": {str(k): round(v, 1) for k, v in tune_ms.items()}},
         "pairs": int(st["pairs"]),
         "phases_ms": {k[3:]: round(v, 3) for k, v in st.items() if k.startswith("ms_")},
-        "per_mesh_rel_err_fro": {"max": max(errs), "mean": float(np.mean(errs))},
     }
+    if errs:
+        out["per_mesh_rel_err_fro"] = {"max": max(errs), "mean": float(np.mean(errs))}
+    if oerrs:
+        out["vs_oracle"] = {"meshes": len(oerrs), "oracle_threads": os.cpu_count() or 1,
+                            "max_rel_err_fro": max(e["rel_err_fro"] for e in oerrs),
+                            "max_abs_err_rel": max(e["max_abs_err_rel"] for e in oerrs),
+                            "patterns_equal": all(e["pattern_equal"] for e in oerrs), "per_mesh": oerrs}
     print(json.dumps(out))
 
 

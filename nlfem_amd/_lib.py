@@ -46,6 +46,8 @@ SIGNATURES = {
                                     C.c_int, _dp]),
     "nlfem_plan_set_estimates": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "nlfem_plan_force_paths": (C.c_int, [C.c_void_p, C.c_int]),
+    "nlfem_plan_set_cut_items": (C.c_int, [C.c_void_p, C.c_int]),
+    "nlfem_plan_autotune": (C.c_int, [C.c_void_p, _ip, _dp]),
     "nlfem_plan_set_surrogate": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp, C.c_float, C.c_double, C.c_double]),
     "nlfem_plan_surrogate_samples": (C.c_int, [C.c_void_p, C.c_longlong, C.c_uint, C.c_uint, C.c_double, C.c_double,
                                                _fp, _fp, _llp]),

@@ -80,6 +80,10 @@ class BatchAssembler:
         self.asm.assemble()
         return self
 
+    def autotune(self):
+        """Pairs per chunk of the cut kernel tuned on this batch (see Assembler.autotune)."""
+        return self.asm.autotune()
+
     def stats(self):
         return self.asm.stats()
 

@@ -2009,7 +2009,8 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
 }
 
 // ======================================================================= cut pairs, two phases
-// One block of 256 threads per outer element aT, cut pairs in chunks of CUT_ITEMS / nPx pairs.
+// One block of 256 threads per outer element aT, cut pairs in chunks of ITEMS / nPx pairs
+// (ITEMS = 64, 128 or 256 items per chunk; the plan can time all three and keep the fastest).
 //   A  one thread per (pair, outer point k): streamed retriangulation; the polygon points go
 //      to an LDS list (at most 9 points, as in the reference's R[18]).
 //   B  one thread per fan triangle (R0, R_j, R_j+1) of any item of the chunk: the inner
@@ -2019,16 +2020,19 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
 //   C  one thread per item: outer accumulation (src/integration.cpp:508-537) into termLocal
 //      (registers) and termNonloc of the pair (LDS).
 //   D  scatter of the chunk's pairs, one thread per (pair, i, j).
-constexpr int CUT_ITEMS = 128;
+constexpr int CUT_ITEMS = 128;    // default items per chunk
+constexpr int CUT_ITEMS_MAX = 256;
 constexpr int CUT_MAXP = 9;
-constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= CUT_ITEMS / CUT_MAXPAIRS = 4
+constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= ITEMS / CUT_MAXPAIRS
 // Dynamic LDS of k_cut: polygons (shared with the pair sums), per-item sums, the outer rule's
 // tables, the chunk's LU factors, and the column map (int keys, 16-bit row positions).
-static size_t cut_lds_bytes(int nPx, int oo, int HN) {
-    const int cp = std::min(CUT_ITEMS / std::max(nPx, 1), CUT_MAXPAIRS);
-    return 16 * (size_t)CUT_ITEMS * CUT_MAXP + 8 * ((size_t)CUT_ITEMS * 10 * oo + (size_t)nPx * (oo + 6)) +
+static size_t cut_lds_bytes(int nPx, int oo, int HN, int items = CUT_ITEMS) {
+    const int cp = std::min(items / std::max(nPx, 1), CUT_MAXPAIRS);
+    return 16 * (size_t)items * CUT_MAXP + 8 * ((size_t)items * 10 * oo + (size_t)nPx * (oo + 6)) +
            sizeof(PivotLU) * (size_t)cp + 10 * (size_t)HN;
 }
+// Whether the outer rule fits a chunk of `items` items (at least one pair, at most CUT_MAXPAIRS).
+static bool cut_items_fit(int nPx, int items) { return nPx <= items && nPx * CUT_MAXPAIRS >= items; }
 
 // Surrogate item (outer point x, inner element E): ZERO when B(x, delta) misses E, FULL when
 // it contains E, PARTIAL otherwise; OUT when the item lies outside the network's domain
@@ -2217,30 +2221,32 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const doubl
 
 // MODE 0 assembles; MODE 1 writes training samples of the surrogate (partial items, exact
 // FP64 inner sums as targets) and assembles nothing.
-template <int KER, int OD, typename T, int MODE = 0>
+template <int KER, int OD, typename T, int MODE = 0, int ITEMS = CUT_ITEMS>
 __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    static_assert(ITEMS == 64 || ITEMS == 128 || ITEMS == 256, "items per chunk: 64, 128 or 256");
+    constexpr int IW = ITEMS / 64;  // wavefronts that hold an item each
     // dynamic LDS, cut_lds_bytes() on the host: the polygons of phases A-B share their space
-    // with the pair sums of phases C-D (CP * 2 NB <= 2 * CUT_ITEMS * CUT_MAXP doubles)
+    // with the pair sums of phases C-D (CP * 2 NB <= 2 * ITEMS * CUT_MAXP doubles)
     const int nPx_ = a.q.nPx;
-    const int CP = min(CUT_ITEMS / nPx_, CUT_MAXPAIRS);             // pairs per chunk
+    const int CP = min(ITEMS / nPx_, CUT_MAXPAIRS);                 // pairs per chunk
     extern __shared__ double ldsc[];
-    double2* pts = (double2*)ldsc;                                  // [CUT_ITEMS][CUT_MAXP]
+    double2* pts = (double2*)ldsc;                                  // [ITEMS][CUT_MAXP]
     double* pacc = ldsc;                                            // [CP][2*NB]: termNonloc, termLocalPrime
-    double* iacc = (double*)(pts + CUT_ITEMS * CUT_MAXP);          // [CUT_ITEMS][10*OO]: innerLocal, innerNonloc, sLP
-    double* tls = iacc + CUT_ITEMS * 10 * OO;                       // [nPx][OO]
+    double* iacc = (double*)(pts + ITEMS * CUT_MAXP);              // [ITEMS][10*OO]: innerLocal, innerNonloc, sLP
+    double* tls = iacc + ITEMS * 10 * OO;                           // [nPx][OO]
     double* xk = tls + nPx_ * OO;                                   // [nPx][2] outer points of aT
     double* wa = xk + 2 * nPx_;                                     // [nPx][3] |aT| dx_k psi_i(x_k)
     double* wd = wa + 3 * nPx_;                                     // [nPx] dx_k |aT|
     PivotLU* lus = (PivotLU*)(wd + nPx_);                           // [CP]
     int* hk = (int*)(lus + CP);                                     // [HN]
     unsigned short* hv = (unsigned short*)(hk + a.HN);              // [HN][3] positions in aT's rows
-    __shared__ int ntri[CUT_ITEMS];
-    __shared__ int tpre[CUT_ITEMS + 1];
+    __shared__ int ntri[ITEMS];
+    __shared__ int tpre[ITEMS + 1];
     __shared__ int pb_[CUT_MAXPAIRS];                               // inner element of each chunk pair
-    __shared__ unsigned char tmap[CUT_ITEMS * (CUT_MAXP - 2)];      // fan triangle -> item
+    __shared__ unsigned char tmap[ITEMS * (CUT_MAXP - 2)];          // fan triangle -> item
     __shared__ unsigned subtri;
-    __shared__ int wsum[2];
+    __shared__ int wsum[4];
     const int aT = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
@@ -2280,7 +2286,7 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
         const int np = min(CP, nB - c0);
         const int nit = np * nPx;
         __syncthreads();
-        for (int i = tid; i < CUT_ITEMS * 10 * OO; i += 256) iacc[i] = 0.0;
+        for (int i = tid; i < ITEMS * 10 * OO; i += 256) iacc[i] = 0.0;
         if (tid < np) {
             const int b = a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + c0 + tid]
                                         : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)] & IDX_MASK);
@@ -2314,20 +2320,20 @@ __global__ __launch_bounds__(256) void k_cut(NumArgs a) {
                 }
             }
             ntri[tid] = nt;
-        } else if (tid < CUT_ITEMS) {
+        } else if (tid < ITEMS) {
             ntri[tid] = 0;
         }
         __syncthreads();
-        {  // inclusive scan of ntri over the first two waves (items < CUT_ITEMS = 128)
-            int v = (tid < CUT_ITEMS) ? ntri[tid] : 0;
+        {  // inclusive scan of ntri over the first IW waves (one item per thread)
+            int v = (tid < ITEMS) ? ntri[tid] : 0;
             for (int off = 1; off < 64; off <<= 1) {
                 const int y = __shfl_up(v, off, 64);
                 if (lane >= off) v += y;
             }
-            if (lane == 63 && w < 2) wsum[w] = v;
+            if (lane == 63 && w < IW) wsum[w] = v;
             __syncthreads();
-            if (w == 1) v += wsum[0];
-            if (tid < CUT_ITEMS) {
+            for (int q = 0; q < w && q < IW; q++) v += wsum[q];
+            if (tid < ITEMS) {
                 tpre[tid + 1] = v;
                 for (int q = v - ntri[tid]; q < v; q++) tmap[q] = (unsigned char)tid;
             }
@@ -3720,6 +3726,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                          \
     } while (0)
     ev_begin(P, PH_CUT);
+    P.cut_items_used = 0;
     bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
@@ -3755,17 +3762,28 @@ void plan_assemble(Plan& P, const Estimates& est) {
             nc.sur.list_in = 1;
         }
         nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
-        const size_t lb = cut_lds_bytes(c.q.nPx, P.oo, nc.HN);
+        // items per chunk of k_cut: the plan's choice when the outer rule and the LDS allow it
+        int items = CUT_ITEMS;
+        if (cut_items_fit(c.q.nPx, P.cut_items) && cut_lds_bytes(c.q.nPx, P.oo, nc.HN, P.cut_items) <= 160 * 1024)
+            items = P.cut_items;
+        const size_t lb = cut_lds_bytes(c.q.nPx, P.oo, nc.HN, items);
         if (lb > 160 * 1024 || P.maxRow > 65535) throw std::length_error("cut-pair LDS exceeds 160 KiB");
-#define LAUNCH_CUT(KER, OD)                                                                                          \
+#define LAUNCH_CUT_I(KER, OD, T, IT)                                                                                  \
     do {                                                                                                             \
-        if (c.precision == 1) {                                                                                      \
-            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER, OD, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
-            k_cut<KER, OD, float><<<c.nE, 256, lb, st>>>(nc);                                                        \
-        } else {                                                                                                     \
-            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER, OD, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
-            k_cut<KER, OD, double><<<c.nE, 256, lb, st>>>(nc);                                                       \
-        }                                                                                                            \
+        if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER, OD, T, 0, IT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb); \
+        k_cut<KER, OD, T, 0, IT><<<c.nE, 256, lb, st>>>(nc);                                                         \
+    } while (0)
+#define LAUNCH_CUT_T(KER, OD, T)                                      \
+    do {                                                             \
+        if (items == 64) LAUNCH_CUT_I(KER, OD, T, 64);               \
+        else if (items == 256) LAUNCH_CUT_I(KER, OD, T, 256);        \
+        else LAUNCH_CUT_I(KER, OD, T, 128);                          \
+    } while (0)
+#define LAUNCH_CUT(KER, OD)                                           \
+    do {                                                             \
+        P.cut_items_used = items;                                    \
+        if (c.precision == 1) LAUNCH_CUT_T(KER, OD, float);          \
+        else LAUNCH_CUT_T(KER, OD, double);                          \
     } while (0)
 #define LAUNCH_CUT8_K(KFN, KER)                                                                           \
     do {                                                                                                  \
@@ -3811,6 +3829,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
             }
         }
 #undef LAUNCH_CUT
+#undef LAUNCH_CUT_T
+#undef LAUNCH_CUT_I
 #undef LAUNCH_CUT8
 #undef LAUNCH_CUT8_K
     }

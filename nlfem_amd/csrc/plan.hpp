@@ -138,6 +138,8 @@ struct Plan {
     std::unordered_map<const void*, size_t> dcap;
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
+    int cut_items = 128;      // items per chunk of the general cut kernel k_cut (64, 128, 256)
+    int cut_items_used = 0;   // items per chunk k_cut ran with in the last assembly (0: k_cut not used)
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
     bool pat_nobm = false;  // the same for the pattern's element set
 

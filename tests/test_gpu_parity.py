@@ -198,3 +198,42 @@ def test_parity_general_paths():
                 paths=flags)
     _check_case("csr-transpose", m, {"function": "parabola", "horizon": 0.1, "outputdim": 1}, make_configuration(),
                 paths=nlfem.Assembler.PATH_TRANSPOSE_CSR)
+
+
+@pytest.mark.parametrize("name", ["C5s", "DG"])
+def test_cut_items_per_chunk_and_autotune(name):
+    """The general cut kernel with 64, 128 and 256 items per chunk, and after the autotune,
+    meets the FP64 bar against the oracle (the chunk size changes summation order only)."""
+    mesh, kernel, conf = case(name)
+    ref = oracle.par_system(mesh, kernel, conf)
+    Abs = oracle.abs_matrix(mesh, kernel, conf).tocsr()
+    floor = 1e-15 * abs(ref["A"]).max()
+
+    def check(A, label):
+        same, rel, _ = compare(A, ref["A"])
+        D = (A - ref["A"]).tocoo()
+        bound = FP64_TOL * np.asarray(Abs[D.row, D.col]).ravel() + floor
+        print(f"[parity] case={name} {label} pattern_equal={same} max_rel={rel:.3e}", flush=True)
+        assert same and np.all(np.abs(D.data) <= bound), label
+
+    for items in (64, 128, 256):
+        asm = nlfem.Assembler(mesh, kernel, conf).set_cut_items(items)
+        try:
+            check(asm.assemble().csr(), f"items={items}")
+        finally:
+            asm.close()
+    asm = nlfem.Assembler(mesh, kernel, conf)
+    try:
+        chosen, ms = asm.autotune()
+        print(f"[parity] case={name} autotune chose {chosen}, cut ms {ms}", flush=True)
+        assert chosen in (64, 128, 256) and ms[128] > 0 and ms[chosen] > 0
+        check(asm.csr(), "after autotune")
+        check(asm.assemble().csr(), f"tuned items={chosen}")
+    finally:
+        asm.close()
+    asm = nlfem.Assembler(mesh, kernel, conf)
+    try:
+        with pytest.raises(nlfem.NlfemError):
+            asm.set_cut_items(100)
+    finally:
+        asm.close()
