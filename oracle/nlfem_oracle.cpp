@@ -18,10 +18,17 @@
 // (b) a contiguous element-block partition instead of METIS_PartMeshDual.  Both only change
 // floating-point summation order.
 //
-// Parity pins (tests/test_oracle_*.py): kernels by closed forms, fully-contained pairs by the
-// closed form of the 7-point rule, A*1 = 0, symmetry, Duffy tables by measure / polynomial
-// exactness, the fractional close-pair routine at s=-1 by an independent polynomial
-// integral, retriangulation by disk-triangle area bounds and hand cases, BFS by brute force.
+// Parity pins (tests/test_oracle*.py): kernels by closed forms, fully-contained pairs by the
+// closed form of the 7-point rule, cut pairs of the polynomial kernels (constant, parabola) by
+// an independent exact quadrature of the same polygons (tests/test_oracle_cutpairs.py),
+// A*1 = 0, symmetry, Duffy tables by measure / polynomial exactness, the fractional
+// close-pair routine at s=-1 by an independent polynomial integral, retriangulation by
+// disk-triangle area bounds and hand cases, BFS by brute force.
+//
+// Parity unpinned: integrate_fractional and integrate_weakSingular at s != -1 (only their
+// s = -1 special case and the Duffy tables they use are checked), and the cut-pair values of
+// the non-polynomial kernels (fractional, microelastic), which share the integration loop that
+// the polynomial kernels pin but whose 7-point sums have no exact value to compare with.
 
 #include <cmath>
 #include <cstdio>
