@@ -251,6 +251,14 @@ def main():
                 v["cut_pairs_by_surrogate"] = int(st2["sur_pairs"])
                 v["cut_pairs_exact_fallback"] = int(st2["sur_rejected"])
                 v["network_items"] = int(st2["sur_items"])
+                # BF16 MFMA work of the two networks (12->64->64->64->9: 9,536 multiply-adds per
+                # item and network) over the whole cut phase, which also runs the exact kernel on
+                # the pairs handed back; the padded per-dispatch count and the counter-based
+                # utilisation are in profiles/mfma__k_cut_sur_h256_delta0.05_constant.md
+                f = st2["sur_items"] * 2 * 2 * 9536
+                v["mfma"] = {"bf16_flops_algorithmic": f, "achieved": round(f / (st2["ms_cut"] / 1000) / 1e12, 2),
+                             "peak": 2500.0, "unit": "TFLOP/s",
+                             "frac": round(f / (st2["ms_cut"] / 1000) / 2.5e15, 4)}
             out["variants"][name] = v
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
