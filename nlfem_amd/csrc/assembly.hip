@@ -1129,7 +1129,11 @@ NLFEM_DEV void kernel_eval(T z0, T z1, const KernelParams& kp, T* K) {
     } else {
         if constexpr (KER == KER_CONSTANT) K[0] = (T)kp.c;
         else if constexpr (KER == KER_PARABOLA) K[0] = (T)kp.c * ((T)kp.sqdelta - (z0 * z0 + z1 * z1));
-        else if constexpr (KER == KER_FRACTIONAL) K[0] = (T)kp.c * ((T)1.0 / pow(sqrt(z0 * z0 + z1 * z1), (T)kp.exponent));
+        // fractional, src/model.cpp:126-138: c / |z|^(2+2s), evaluated as c 2^(-(1+s) log2 |z|^2)
+        // (FP64: a few ulps from the reference's pow of the square root)
+        else if constexpr (KER == KER_FRACTIONAL && F64) K[0] = kp.c * exp2(-0.5 * kp.exponent * log2(z0 * z0 + z1 * z1));
+        else if constexpr (KER == KER_FRACTIONAL)
+            K[0] = (T)kp.c * exp2f((float)(-0.5 * kp.exponent) * log2f(z0 * z0 + z1 * z1));
         else if constexpr (F64) K[0] = kp.c * rsqrt_nr(z0 * z0 + z1 * z1);
         else K[0] = (T)kp.c * ((T)1.0 / sqrt(z0 * z0 + z1 * z1));
     }
