@@ -111,10 +111,13 @@ int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double 
  * numbering is not local or a fixed-size layout overflows (no counterpart in the reference;
  * for checking those paths and for meshes whose numbering is known to be poor).  Bits:
  * NLFEM_PATH_BFS_HASH: hashed BFS visited sets; NLFEM_PATH_PATTERN_HASH: hashed pattern
- * element sets; NLFEM_PATH_TRANSPOSE_CSR: transposed relation counted into exact CSR rows. */
+ * element sets; NLFEM_PATH_TRANSPOSE_CSR: transposed relation counted into exact CSR rows;
+ * NLFEM_PATH_BFS_WAVE: one outer element per wavefront in the bitmap BFS (by default two
+ * outer elements share a wavefront, 32 lanes each). */
 #define NLFEM_PATH_BFS_HASH 1
 #define NLFEM_PATH_PATTERN_HASH 2
 #define NLFEM_PATH_TRANSPOSE_CSR 4
+#define NLFEM_PATH_BFS_WAVE 8
 int nlfem_plan_force_paths(nlfem_plan* plan, int flags);
 /* Items (pair x outer point) per chunk of the general cut kernel, which serves outdim 2, DG
  * and outer rules of more than 8 points: 64, 128 (default) or 256, i.e. 4, 8 or 16 pairs per

@@ -653,6 +653,218 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
 }
 
+// Two outer elements per wavefront: half-wavefront h (lanes 32 h .. 32 h + 31) runs the BFS of
+// outer element 2 blockIdx.x + h with the bitmap visited set, exactly as k_bfs does with a
+// whole wavefront (same tests, same expansion, same list layout).  The BFS is bound by the
+// latency of its dependent levels, not by arithmetic, and its register demand limits the
+// resident wavefronts, so two searches per wavefront double the searches in flight.  Control
+// flow stays wavefront-uniform: loops run to the larger of the two halves' bounds with per-half
+// predicates, counters are half-uniform, and ranks come from ballots masked to the half.
+// The level buffers hold 16-bit offsets into the bitmap window (every element stored there
+// has passed the window test), which needs a window of at most 65,536 elements.
+NLFEM_DEV unsigned long long half_mask() { return (__lane_id() & 32) ? 0xffffffff00000000ull : 0xffffffffull; }
+NLFEM_DEV int half_rank(bool p, int& total) {
+    const unsigned long long m = __ballot(p) & half_mask();
+    total = __popcll(m);
+    return __popcll(m & ((1ull << __lane_id()) - 1ull));
+}
+NLFEM_DEV bool half_any(bool p) { return (__ballot(p) & half_mask()) != 0ull; }
+NLFEM_DEV int wave_max2(int v) { return max(__shfl(v, 0, 64), __shfl(v, 32, 64)); }
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_bfs_h(BfsArgs a) {
+    extern __shared__ int ldsh[];
+    __shared__ OuterPts ops[2];
+    const int lane = threadIdx.x;
+    const int hf = lane >> 5, hl = lane & 31;
+    int* hash = ldsh + hf * (a.H + a.F);
+    unsigned short* test = reinterpret_cast<unsigned short*>(hash + a.H);
+    unsigned short* next = test + a.F;
+    OuterPts& op = ops[hf];
+    const int aT = 2 * blockIdx.x + hf;
+    const bool live = aT < a.nE;
+    const int aTs = live ? aT : 0;
+    const ElemRec A = a.elem[aTs];
+    const bool skip = !live || A.label == 0;
+    const int base = aT - a.whalf;  // element e is bit / offset e - base of the window
+    if (live && A.label == 0 && hl == 0) { a.cntF[aT] = 0; a.cntB[aT] = 0; }
+    for (int i = hl; i < a.H; i += 32) hash[i] = 0;
+    if (hl == 0 && !skip) outer_points(a, A, op);
+    __syncthreads();
+    int nTest = 0, nNext = 0, nTot = 0, nF = 0, nB = 0;
+    bool ovf = false, wovf = false;
+    unsigned cProbe = 0, cClose = 0, cInt = 0, cCut = 0, cBary = 0;  // per lane: far below 2^32
+    // level 0 candidates: aT itself, then its dual-graph neighbours
+    const long long x0 = a.xadj[aTs];
+    const int deg = skip ? 0 : (int)(a.xadj[aTs + 1] - x0);
+    bool lo = false;
+    if (!skip && 1 + deg <= a.F) {
+        if (hl == 0) { visit_insert<true>(hash, a, aT, aT); test[0] = (unsigned short)a.whalf; }
+        for (int j = hl; j < deg; j += 32) {
+            const int b = a.adj[x0 + j];
+            const bool inw = visit_insert<true>(hash, a, aT, b) >= 0;
+            if (!inw) lo = true;
+            test[1 + j] = inw ? (unsigned short)(b - base) : (unsigned short)0;
+        }
+    }
+    lo = half_any(lo);
+    ovf = (!skip && 1 + deg > a.F) || lo;
+    wovf = lo;
+    nTest = skip ? 0 : 1 + deg;
+    __syncthreads();
+
+    auto record = [&](int b, int cls, bool push) {
+        const bool rec = cls >= 0;
+        const bool isF = cls == (int)CLS_INTERIOR || cls == (int)CLS_CLOSE;
+        int nr, nf, nb;
+        half_rank(rec, nr);
+        const int rf = half_rank(rec && isF, nf);
+        const int rb = half_rank(rec && !isF, nb);
+        if (nTot + nr > a.cap) ovf = true;
+        if (rec && !ovf) {
+            const int slot = isF ? nF + rf : a.cap - 1 - (nB + rb);
+            const uint32_t code = (cls == 3) ? CLS_CUT : (uint32_t)cls;
+            a.nbr[(size_t)aT * a.cap + slot] = (uint32_t)b | (code << CLS_SHIFT) | (cls == 3 ? (1u << 31) : 0u);
+            if (cls == (int)CLS_CLOSE) {
+                const unsigned long long c = atomicAdd(&a.counters[CNT_CLOSE], 1ull);
+                if ((long long)c < a.closeCap) a.closeList[c] = make_int2(aT, b);
+            }
+        }
+        nTot += nr; nF += nf; nB += nb;
+        cClose += cls == (int)CLS_CLOSE; cInt += cls == (int)CLS_INTERIOR; cCut += cls == (int)CLS_CUT;
+        cBary += cls == 3;
+        const bool q = push && b != aT;
+        int nq;
+        const int rq = half_rank(q, nq);
+        if (nNext + nq > a.F) ovf = true;
+        else if (q) next[nNext + rq] = (unsigned short)(b - base);
+        nNext += nq;
+    };
+
+    bool done = skip || ovf;
+    int level = 0;
+    while (__any(!done)) {
+        const int method = (level == 0) ? a.level0 : a.remote;
+        int nHard = 0;
+        const int nT = done ? 0 : nTest;
+        const int mT = wave_max2(nT);
+        for (int r0 = 0; r0 < mT; r0 += 32) {
+            const int j = r0 + hl;
+            const bool valid = j < nT;
+            const int b = valid ? base + (int)test[j] : -1;
+            int cls = -1;
+            bool push = false, hard = false;
+            if (valid) {
+                const ElemRec B = a.elem[b];
+                if (B.label == 0) {
+                    push = true;  // label-0 elements are traversed only (src/Cassemble.cpp:773-775)
+                } else {
+                    cls = classify_quick(a, op, B, method);
+                    hard = cls == CLS_HARD;
+                    if (hard) cls = -1;
+                    push = cls >= 0;
+                    cProbe += !hard && cls < 0;
+                }
+            }
+            int nh;
+            const int rh = half_rank(hard, nh);
+            if (hard) test[nHard + rh] = (unsigned short)(b - base);
+            nHard += nh;
+            record(b, cls, push);
+        }
+        __syncthreads();
+        const int mH = wave_max2(nHard);
+        for (int r0 = 0; r0 < mH; r0 += 32) {
+            const int j = r0 + hl;
+            const bool valid = j < nHard;
+            const int b = valid ? base + (int)test[j] : -1;
+            int cls = -1;
+            if (valid) {
+                cls = classify_full(a, op, a.elem[b]);
+                cProbe += cls < 0;
+            }
+            record(b, cls, cls >= 0);
+        }
+        ovf = half_any(ovf);
+        __syncthreads();
+        if (nNext == 0 || ovf) done = true;
+        if (!__any(!done)) break;
+        // discover the next level: unvisited neighbours of the expanded elements
+        nTest = 0;
+        bool lovf = false;
+        auto append = [&](int e, bool isnew) {
+            int nt;
+            const int rt = half_rank(isnew, nt);
+            if (isnew && nTest + rt < a.F) test[nTest + rt] = (unsigned short)(e - base);
+            nTest += nt;
+        };
+        const int nN = done ? 0 : nNext;
+        const int mN = wave_max2(nN);
+        for (int r0 = 0; r0 < mN; r0 += 32) {
+            const int j = r0 + hl;
+            const int s = j < nN ? base + (int)next[j] : -1;
+            const int4* row = reinterpret_cast<const int4*>(a.adjp) + (size_t)(s < 0 ? 0 : s) * a.W4;
+            int4 v[BFS_W4MAX];
+#pragma unroll
+            for (int c = 0; c < BFS_W4MAX; c++)
+                if (c < a.W4) v[c] = s >= 0 ? row[c] : make_int4(-1, -1, -1, -1);
+            int ev[4 * BFS_W4MAX];
+            unsigned old[4 * BFS_W4MAX];
+#pragma unroll
+            for (int c = 0; c < BFS_W4MAX; c++) {
+                ev[4 * c] = v[c].x; ev[4 * c + 1] = v[c].y; ev[4 * c + 2] = v[c].z; ev[4 * c + 3] = v[c].w;
+            }
+#pragma unroll
+            for (int q = 0; q < 4 * BFS_W4MAX; q++) {
+                old[q] = 0xffffffffu;
+                if (q < 4 * a.W4 && ev[q] >= 0) {
+                    const int idx = ev[q] - base;
+                    if ((unsigned)idx >= (unsigned)(32 * a.H)) lovf = true;
+                    else old[q] = atomicOr((unsigned*)&hash[idx >> 5], 1u << (idx & 31));
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4 * BFS_W4MAX; q++) {
+                if (q >= 4 * a.W4) break;
+                const int idx = ev[q] - base;
+                const bool isnew = ev[q] >= 0 && !((old[q] >> (idx & 31)) & 1u);
+                append(ev[q], isnew);
+            }
+        }
+        const bool al = half_any(lovf);
+        if (al) wovf = true;
+        if (!done) {
+            ovf = al || nTest > a.F;
+            if (ovf) done = true;
+        }
+        nNext = 0;
+        __syncthreads();
+        level++;
+    }
+    for (int m = 16; m > 0; m >>= 1) {
+        cProbe += __shfl_xor(cProbe, m, 64);
+        cClose += __shfl_xor(cClose, m, 64);
+        cInt += __shfl_xor(cInt, m, 64);
+        cCut += __shfl_xor(cCut, m, 64);
+        cBary += __shfl_xor(cBary, m, 64);
+    }
+    if (hl == 0 && !skip) {
+        if (ovf) {
+            atomicOr(&a.counters[CNT_OVERFLOW], wovf ? 2ull : 1ull);
+            a.cntF[aT] = 0;
+            a.cntB[aT] = 0;
+        } else {
+            a.cntF[aT] = nF;
+            a.cntB[aT] = nB;
+            atomicMax(&a.counters[CNT_MAXLIST], (unsigned long long)(nF + nB));
+        }
+        atomicAdd(&a.counters[CNT_PAIRS], (unsigned long long)cClose + cInt + cCut + cBary);
+        atomicAdd(&a.counters[CNT_PROBES], (unsigned long long)cProbe);
+        atomicAdd(&a.counters[CNT_INTERIOR], (unsigned long long)cInt);
+        atomicAdd(&a.counters[CNT_CUT], (unsigned long long)cCut);
+        atomicAdd(&a.counters[CNT_BARY], (unsigned long long)cBary);
+    }
+}
+
 NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
 
 // ======================================================================= transpose
@@ -3433,7 +3645,11 @@ void plan_assemble(Plan& P, const Estimates& est) {
         long long H = next_pow2(std::min<long long>(std::max<long long>(hvis, 64), 2 * cap));
         long long F = std::min<long long>(next_pow2(std::max<long long>(fvis, 256)), next_pow2(cap));
         if (use_bm) H = bm_words;
-        size_t ldsb = sizeof(int) * (size_t)(H + 2 * F);
+        // two outer elements per wavefront (k_bfs_h) with the bitmap when both searches' tables
+        // stay small
+        // (16-bit level buffers: a window of at most 65,536 elements)
+        bool half = use_bm && !P.bfs_wave && 32 * H <= 65536 && 2 * sizeof(int) * (size_t)(H + F) <= 64 * 1024;
+        size_t ldsb = half ? 2 * sizeof(int) * (size_t)(H + F) : sizeof(int) * (size_t)(H + 2 * F);
         if (ldsb > 160 * 1024) {
             // too large for LDS: fall back to the largest table that fits and rely on overflow detection
             H = use_bm ? H : 16384;
@@ -3448,7 +3664,10 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ba.counters = P.d_counters;
         HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
         ev_begin(P, PH_BFS);
-        if (use_bm) {
+        if (half) {
+            if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs_h, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+            k_bfs_h<<<(c.nE + 1) / 2, 64, ldsb, st>>>(ba);
+        } else if (use_bm) {
             if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
             k_bfs<true><<<c.nE, 64, ldsb, st>>>(ba);
         } else {

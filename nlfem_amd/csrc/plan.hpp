@@ -141,6 +141,7 @@ struct Plan {
     int cut_items = 128;      // items per chunk of the general cut kernel k_cut (64, 128, 256)
     int cut_items_used = 0;   // items per chunk k_cut ran with in the last assembly (0: k_cut not used)
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
+    bool bfs_wave = false;  // one outer element per wavefront in the bitmap BFS (k_bfs instead of k_bfs_h)
     bool pat_nobm = false;  // the same for the pattern's element set
 
     Stats stats;

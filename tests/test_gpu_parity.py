@@ -198,6 +198,8 @@ def test_parity_general_paths():
                 paths=flags)
     _check_case("csr-transpose", m, {"function": "parabola", "horizon": 0.1, "outputdim": 1}, make_configuration(),
                 paths=nlfem.Assembler.PATH_TRANSPOSE_CSR)
+    _check_case("bfs-wave", m, {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration(),
+                paths=nlfem.Assembler.PATH_BFS_WAVE)
 
 
 @pytest.mark.parametrize("name", ["C5s", "DG"])
