@@ -867,6 +867,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 
 NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
 
+// Block index with the XCD-major order: workgroups are dispatched to the 8 XCDs round-robin
+// (hardware block i runs on XCD i % 8), so the hardware blocks of one XCD take one contiguous
+// range of logical blocks.  Neighbouring outer elements write into the same rows of shared
+// arrays; keeping them on one XCD lets those partial lines merge in that XCD's L2 instead of
+// being written back by several L2s.
+constexpr int NXCD = 8;
+NLFEM_DEV int xcd_block(int b, int nb) {
+    const int q = nb / NXCD, r = nb % NXCD, x = b % NXCD, slot = b / NXCD;
+    return x * q + min(x, r) + slot;
+}
+
 // ======================================================================= transpose
 // Transposed relation.  Row b lists the outer elements aT of the pairs (aT, b): pairs from
 // the front segments (interior, close) from the row's start, cut pairs from its end.  The cut
@@ -899,23 +910,51 @@ NLFEM_DEV int lds_find_b(const int* h, int key) {
 // FILL = false: tcnt[b] += entries of row b.  FILL = true: place the entries; tfill / tfillB
 // count the front / back entries placed so far (they end as row b's front and cut-entry
 // counts).  Row b occupies [tptr[b], tptr[b+1]): the exact CSR rows of a counting pass, or
-// padded rows of a fixed capacity, whose overflow k_tcheck reports.
-template <bool FILL>
+// padded rows of capT entries (row b at b capT; capT = 0: CSR rows), whose overflow k_tcheck
+// reports.  PK (padded rows with capT < 32768): the block's front and back counts of a row
+// share one LDS word (16 bits each), so the tables take 32 KB instead of 48 KB; a row whose
+// counts would not fit 16 bits holds more than capT entries, which k_tcheck reports, and the
+// plan redoes the transpose with CSR rows.
+template <bool FILL, bool PK = false>
 __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
                                                    const int* cntB, int* tcnt, const long long* tptr, int* tfill,
-                                                   int* tadj, const long long* coff, int* tfillB, int* tcut) {
-    __shared__ int hk[TH], hf[TH], hb[TH];
+                                                   int* tadj, const long long* coff, int* tfillB, int* tcut,
+                                                   int capT) {
+    static_assert(!PK || FILL, "packed counts only in the fill pass");
+    __shared__ int sh[PK ? 2 * TH : 3 * TH];
+    int* hk = sh;
+    int* hf = sh + TH;
+    int* hb = PK ? hf : sh + 2 * TH;
+    constexpr int BK = PK ? 0x10000 : 1;  // one back entry in hb
     const int tid = threadIdx.x;
     for (int i = tid; i < TH; i += 256) { hk[i] = -1; hf[i] = 0; hb[i] = 0; }
     __syncthreads();
-    const int a0 = blockIdx.x * tg, a1 = min(a0 + tg, nE);
+    const int a0 = xcd_block(blockIdx.x, gridDim.x) * tg, a1 = min(a0 + tg, nE);
+    // entries in groups of TU per thread: the group's list loads are issued before its LDS
+    // updates, so the loads of one group overlap
+    constexpr int TU = 4;
     for (int aT = a0; aT < a1; aT++) {
         const int nF = cntF[aT], n = nF + cntB[aT];
-        for (int p = tid; p < n; p += 256) {
-            const int b = (int)(nbr[(size_t)aT * cap + list_index(p, nF, cap)] & IDX_MASK);
-            const int sl = lds_slot_b(hk, b);
-            if (sl >= 0) atomicAdd(p < nF || !FILL ? &hf[sl] : &hb[sl], 1);
-            else if (!FILL) atomicAdd(&tcnt[b], 1);
+        for (int p0 = tid; p0 < n; p0 += TU * 256) {
+            uint32_t ev[TU];
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int p = p0 + u * 256;
+                ev[u] = p < n ? nbr[(size_t)aT * cap + list_index(p, nF, cap)] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int p = p0 + u * 256;
+                if (p >= n) break;
+                const int b = (int)(ev[u] & IDX_MASK);
+                const int sl = lds_slot_b(hk, b);
+                if (sl >= 0) {
+                    if (p < nF || !FILL) atomicAdd(&hf[sl], 1);
+                    else atomicAdd(&hb[sl], BK);
+                } else if (!FILL) {
+                    atomicAdd(&tcnt[b], 1);
+                }
+            }
         }
     }
     __syncthreads();
@@ -923,7 +962,12 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
         const int b = hk[i];
         if (b < 0) continue;
         if (!FILL) atomicAdd(&tcnt[b], hf[i]);
-        else {  // reserve the block's ranges; hf / hb become the next free rank
+        else if (PK) {  // reserve the block's ranges; the word becomes the next free ranks
+            const int v = hf[i], f = v & 0xffff, k = (unsigned)v >> 16;
+            const int rf = f ? atomicAdd(&tfill[b], f) : 0;
+            const int rb = k ? atomicAdd(&tfillB[b], k) : 0;
+            hf[i] = (rf & 0xffff) | (rb << 16);
+        } else {
             if (hf[i]) hf[i] = atomicAdd(&tfill[b], hf[i]);
             if (hb[i]) hb[i] = atomicAdd(&tfillB[b], hb[i]);
         }
@@ -932,23 +976,40 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
     __syncthreads();
     for (int aT = a0; aT < a1; aT++) {
         const int nF = cntF[aT], n = nF + cntB[aT];
-        for (int p = tid; p < n; p += 256) {
-            const uint32_t e = nbr[(size_t)aT * cap + list_index(p, nF, cap)];
+        const long long co = coff[aT];
+        for (int p0 = tid; p0 < n; p0 += TU * 256) {
+          uint32_t ev[TU];
+#pragma unroll
+          for (int u = 0; u < TU; u++) {
+              const int p = p0 + u * 256;
+              ev[u] = p < n ? nbr[(size_t)aT * cap + list_index(p, nF, cap)] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < TU; u++) {
+            const int p = p0 + u * 256;
+            if (p >= n) break;
+            const uint32_t e = ev[u];
             const int b = (int)(e & IDX_MASK);
             const int sl = lds_find_b(hk, b);
             long long q;
-            const long long room = tptr[b + 1] - tptr[b];  // the row's capacity
+            const long long r0 = capT ? (long long)b * capT : tptr[b];
+            const long long r1 = capT ? r0 + capT : tptr[b + 1];
+            const long long room = r1 - r0;  // the row's capacity
             if (p < nF) {
-                const long long r = sl >= 0 ? atomicAdd(&hf[sl], 1) : atomicAdd(&tfill[b], 1);
+                const long long r = sl >= 0 ? (long long)(atomicAdd(&hf[sl], 1) & (PK ? 0xffff : -1))
+                                            : atomicAdd(&tfill[b], 1);
                 if (r >= room) continue;  // padded rows: overflow, detected by k_tcheck
-                q = tptr[b] + r;
+                q = r0 + r;
             } else {
-                const long long r = sl >= 0 ? atomicAdd(&hb[sl], 1) : atomicAdd(&tfillB[b], 1);
+                const long long r = sl >= 0 ? (long long)(PK ? ((unsigned)atomicAdd(&hb[sl], BK) >> 16)
+                                                              : (unsigned)atomicAdd(&hb[sl], 1))
+                                            : atomicAdd(&tfillB[b], 1);
                 if (r >= room) continue;
-                q = tptr[b + 1] - 1 - r;
-                tcut[q] = (int)(coff[aT] + (p - nF));
+                q = r1 - 1 - r;
+                tcut[q] = (int)(co + (p - nF));
             }
             tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
+          }
         }
     }
 }
@@ -3726,8 +3787,13 @@ void plan_assemble(Plan& P, const Estimates& est) {
         k_tptr_padded<<<nblk(c.nE + 1, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tptr);
         HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
         HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
-        k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
-                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
+        if (P.cap < 32768)
+            k_transpose<true, true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
+                                                             P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut,
+                                                             P.cap);
+        else
+            k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
+                                                       P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, P.cap);
         k_tcheck<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tF, P.d_tcB, P.d_counters);
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
@@ -3742,7 +3808,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     if (!tdone) {
         HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
         k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
-                                                    nullptr, nullptr, nullptr, nullptr, nullptr);
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr, 0);
         scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
         long long ntadj = 0;
         HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -3752,7 +3818,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
         HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
         k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
-                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut);
+                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, 0);
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);
