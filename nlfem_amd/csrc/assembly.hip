@@ -2116,30 +2116,50 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     // T as outer element: front segment, interior class
     const int nF = a.cntF[T0];
     if constexpr (FACT) {
-        for (int p = tid; p < nF; p += 256) {
-            const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
-            if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
-            const int U = (int)(e & IDX_MASK);
-            const double w = a.kp.c * A.absDet * a.elem[U].absDet;
-            sumB += a.elem[U].absDet;
-            for (int j = 0; j < 3; j++) {  // column U_j collects w q1_j; row T_i takes p1_i of it below
-                const int s = cm.find(node_of(a.tri, U, j, a.is_DG));
-                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
-                atomicAdd(&acc[s * 3 + 0], w * a.q1[j]);
-            }
-        }
-        const long long tb = a.tptr[T0], te = tb + a.tF[T0];  // interior entries are front entries
-        for (long long q = tb + tid; q < te; q += 256) {
-            const uint32_t e = (uint32_t)a.tadj[q];
-            if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
-            const int U = (int)(e & IDX_MASK);
-            const double absU = a.elem[U].absDet;
-            const double w = a.kp.c * absU * A.absDet;
-            sumU += absU;
-            for (int i = 0; i < 3; i++) {  // -termNonlocPrime into (T_j, U_i): column U_i collects w p1_i
-                const int s = cm.find(node_of(a.tri, U, i, a.is_DG));
-                if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
-                atomicAdd(&acc[s * 3 + 1], w * a.p1[i]);
+        // Pairs in groups of IU per thread: the group's list entries, then its elements' |det| and
+        // nodes, are loaded before any LDS update, so the dependent loads of a group overlap.
+        // Role 0, T as outer element (front segment of its list): column U_j collects w q1_j,
+        // row T_i takes p1_i of it below.  Role 1, T as inner element (front entries of its
+        // transposed row): -termNonlocPrime into (T_j, U_i), column U_i collects w p1_i.
+        constexpr int IU = 4;
+        const long long tb = a.tptr[T0];
+        const int nT = a.tF[T0];
+#pragma unroll
+        for (int role = 0; role < 2; role++) {
+            const int n = role == 0 ? nF : nT;
+            for (int p0 = tid; p0 < n; p0 += IU * 256) {
+                int Us[IU];
+#pragma unroll
+                for (int u = 0; u < IU; u++) {
+                    const int p = p0 + u * 256;
+                    Us[u] = -1;
+                    if (p < n) {
+                        const uint32_t e = role == 0 ? a.nbr[(size_t)T0 * a.cap + p] : (uint32_t)a.tadj[tb + p];
+                        if (!(e >> 31) && ((e >> CLS_SHIFT) & 3u) == CLS_INTERIOR) Us[u] = (int)(e & IDX_MASK);
+                    }
+                }
+                double dU[IU];
+                int nd[IU][3];
+#pragma unroll
+                for (int u = 0; u < IU; u++) {
+                    const int U = Us[u] < 0 ? T0 : Us[u];
+                    dU[u] = a.elem[U].absDet;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) nd[u][j] = node_of(a.tri, U, j, a.is_DG);
+                }
+#pragma unroll
+                for (int u = 0; u < IU; u++) {
+                    if (Us[u] < 0) continue;
+                    const double w = role == 0 ? a.kp.c * A.absDet * dU[u] : a.kp.c * dU[u] * A.absDet;
+                    if (role == 0) sumB += dU[u];
+                    else sumU += dU[u];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int s = cm.find(nd[u][j]);
+                        if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+                        atomicAdd(&acc[s * 3 + role], w * (role == 0 ? a.q1[j] : a.p1[j]));
+                    }
+                }
             }
         }
         __syncthreads();
