@@ -400,6 +400,19 @@ NLFEM_DEV int classify_full(const BfsArgs& a, const OuterPts& op, const ElemRec&
     return a.fullCC ? (int)CLS_CUT : -1;
 }
 
+// Block index with the XCD-major order: workgroups are dispatched to the 8 XCDs round-robin
+// (hardware block i runs on XCD i % 8), so the hardware blocks of one XCD take one contiguous
+// range of logical blocks.  Neighbouring outer elements write into the same rows of shared
+// arrays; keeping them on one XCD lets those partial lines merge in that XCD's L2 instead of
+// being written back by several L2s.  (The quadrature kernels and the BFS keep the
+// round-robin order, which spreads the spatially uneven work better: measured slower with
+// this order.)
+constexpr int NXCD = 8;
+NLFEM_DEV int xcd_block(int b, int nb) {
+    const int q = nb / NXCD, r = nb % NXCD, x = b % NXCD, slot = b / NXCD;
+    return x * q + min(x, r) + slot;
+}
+
 // Padded dual-graph rows of up to 4 * BFS_W4MAX entries are loaded at once; longer rows use the CSR.
 constexpr int BFS_W4MAX = 4;
 
@@ -866,17 +879,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 }
 
 NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
-
-// Block index with the XCD-major order: workgroups are dispatched to the 8 XCDs round-robin
-// (hardware block i runs on XCD i % 8), so the hardware blocks of one XCD take one contiguous
-// range of logical blocks.  Neighbouring outer elements write into the same rows of shared
-// arrays; keeping them on one XCD lets those partial lines merge in that XCD's L2 instead of
-// being written back by several L2s.
-constexpr int NXCD = 8;
-NLFEM_DEV int xcd_block(int b, int nb) {
-    const int q = nb / NXCD, r = nb % NXCD, x = b % NXCD, slot = b / NXCD;
-    return x * q + min(x, r) + slot;
-}
 
 // ======================================================================= transpose
 // Transposed relation.  Row b lists the outer elements aT of the pairs (aT, b): pairs from
