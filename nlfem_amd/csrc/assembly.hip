@@ -705,7 +705,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __syncthreads();
     int nTest = 0, nNext = 0, nTot = 0, nF = 0, nB = 0;
     bool ovf = false, wovf = false;
-    unsigned cProbe = 0, cClose = 0, cInt = 0, cCut = 0, cBary = 0;  // per lane: far below 2^32
+    // per-lane pair counts (each far below 2^16): probes; interior | close << 16; cut | bary << 16
+    unsigned cProbe = 0, cIC = 0, cCB = 0;
     // level 0 candidates: aT itself, then its dual-graph neighbours
     const long long x0 = a.xadj[aTs];
     const int deg = skip ? 0 : (int)(a.xadj[aTs + 1] - x0);
@@ -743,8 +744,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
         }
         nTot += nr; nF += nf; nB += nb;
-        cClose += cls == (int)CLS_CLOSE; cInt += cls == (int)CLS_INTERIOR; cCut += cls == (int)CLS_CUT;
-        cBary += cls == 3;
+        cIC += (cls == (int)CLS_INTERIOR ? 1u : 0u) + (cls == (int)CLS_CLOSE ? 0x10000u : 0u);
+        cCB += (cls == (int)CLS_CUT ? 1u : 0u) + (cls == 3 ? 0x10000u : 0u);
         const bool q = push && b != aT;
         int nq;
         const int rq = half_rank(q, nq);
@@ -853,8 +854,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         __syncthreads();
         level++;
     }
+    unsigned long long cInt = cIC & 0xffffu, cClose = cIC >> 16, cCut = cCB & 0xffffu, cBary = cCB >> 16;
+    unsigned long long cPr = cProbe;
     for (int m = 16; m > 0; m >>= 1) {
-        cProbe += __shfl_xor(cProbe, m, 64);
+        cPr += __shfl_xor(cPr, m, 64);
         cClose += __shfl_xor(cClose, m, 64);
         cInt += __shfl_xor(cInt, m, 64);
         cCut += __shfl_xor(cCut, m, 64);
@@ -870,11 +873,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             a.cntB[aT] = nB;
             atomicMax(&a.counters[CNT_MAXLIST], (unsigned long long)(nF + nB));
         }
-        atomicAdd(&a.counters[CNT_PAIRS], (unsigned long long)cClose + cInt + cCut + cBary);
-        atomicAdd(&a.counters[CNT_PROBES], (unsigned long long)cProbe);
-        atomicAdd(&a.counters[CNT_INTERIOR], (unsigned long long)cInt);
-        atomicAdd(&a.counters[CNT_CUT], (unsigned long long)cCut);
-        atomicAdd(&a.counters[CNT_BARY], (unsigned long long)cBary);
+        atomicAdd(&a.counters[CNT_PAIRS], cClose + cInt + cCut + cBary);
+        atomicAdd(&a.counters[CNT_PROBES], cPr);
+        atomicAdd(&a.counters[CNT_INTERIOR], cInt);
+        atomicAdd(&a.counters[CNT_CUT], cCut);
+        atomicAdd(&a.counters[CNT_BARY], cBary);
     }
 }
 
