@@ -22,13 +22,14 @@
 // closed form of the 7-point rule, cut pairs of the polynomial kernels (constant, parabola) by
 // an independent exact quadrature of the same polygons (tests/test_oracle_cutpairs.py),
 // A*1 = 0, symmetry, Duffy tables by measure / polynomial exactness, the close-pair routines
-// at s=-1 by an independent polynomial integral, integrate_fractional on an identical pair at
-// s=0.4 by its convergence to a semi-analytic value (tests/test_oracle_fractional.py),
+// at s=-1 by an independent polynomial integral, integrate_fractional at s=0.4 on an identical
+// pair by its convergence to a semi-analytic value and on vertex- and edge-sharing pairs
+// against a direct quadrature, to the quadrature error of ~3e-4 (tests/test_oracle_fractional.py),
 // retriangulation by disk-triangle area bounds and hand cases, BFS by brute force.
 //
-// Parity unpinned: integrate_fractional on vertex- and edge-sharing pairs at s != -1, and
-// integrate_weakSingular at s != -1 (their s = -1 special case and the Duffy tables they use
-// are checked), and the cut-pair values of the non-polynomial kernels (fractional,
+// Parity unpinned: integrate_weakSingular at s != -1 (its s = -1 special case and the Duffy
+// tables it uses are checked), the touching pairs of integrate_fractional below the ~3e-4 of
+// their check, and the cut-pair values of the non-polynomial kernels (fractional,
 // microelastic), which share the integration loop that the polynomial kernels pin but whose
 // 7-point sums have no exact value to compare with.
 

@@ -99,3 +99,65 @@ def test_fractional_identical_pair_converges_to_the_semi_analytic_value():
     # the same test fails for a plausible mistake: the exponent 2 + s instead of 2 + 2s
     wrong = _identical_pair_exact(E, 0.2, c)
     assert np.abs(_oracle_single(E, s, delta, 12) - wrong).max() / scale > 10 * err[12]
+
+
+def _collapsed(m):
+    g, w = np.polynomial.legendre.leggauss(m)
+    t, wt = 0.5 * (g + 1), 0.5 * w
+    U, V = np.meshgrid(t, t, indexing="ij")
+    return np.stack([(U * (1 - V)).ravel(), V.ravel()], 1), (np.outer(wt, wt) * (1 - V)).ravel()
+
+
+def _cross_direct(EX, NX, EY, NY, nV, s, c, m):
+    """c int_X int_Y (u(x)-u(y)) (v(x)-v(y)) |x-y|^(-2-2s) for two different triangles that share
+    a vertex or an edge, by a product of collapsed Gauss rules (interior points never meet)."""
+    P, W = _collapsed(m)
+    X = EX[0] + P[:, :1] * (EX[1] - EX[0]) + P[:, 1:] * (EX[2] - EX[0])
+    Y = EY[0] + P[:, :1] * (EY[1] - EY[0]) + P[:, 1:] * (EY[2] - EY[0])
+    dX = abs(np.linalg.det(np.array([EX[1] - EX[0], EX[2] - EX[0]])))
+    dY = abs(np.linalg.det(np.array([EY[1] - EY[0], EY[2] - EY[0]])))
+    phi = np.column_stack([1 - P[:, 0] - P[:, 1], P[:, 0], P[:, 1]])
+    UX = np.zeros((len(P), nV))
+    UY = np.zeros((len(P), nV))
+    for a in range(3):
+        UX[:, NX[a]] += phi[:, a]
+        UY[:, NY[a]] += phi[:, a]
+    K = np.linalg.norm(X[:, None, :] - Y[None, :, :], axis=2) ** (-2 - 2 * s)
+    D = UX[:, None, :] - UY[None, :, :]
+    return c * dX * dY * np.einsum("p,q,pq,pqi,pqj->ij", W, W, K, D, D)
+
+
+@pytest.mark.parametrize("layout,tol", [("vertex", 4e-4), ("edge", 6e-4)])
+def test_fractional_touching_pairs_at_s_04(layout, tol):
+    """Two triangles sharing a vertex or an edge at s = 0.4: the oracle's matrix at tensor-Gauss
+    degree 12 against the identical-pair values above plus a direct product rule (32^2 x 32^2)
+    for the two cross pairs.  The direct rule itself is accurate to ~1e-5 (vertex) and ~2e-4
+    (edge, singular along the whole shared edge); the oracle's degree-12 rule to ~2e-4.  A cross
+    pair dropped, or counted with the wrong sign, is off by order one."""
+    if layout == "vertex":
+        verts = np.array([[0.0, 0.0], [0.1, 0.0], [0.0, 0.1], [-0.1, 0.02], [-0.03, -0.1]])
+        elems = np.array([[0, 1, 2], [0, 3, 4]])
+    else:
+        verts = np.array([[0.0, 0.0], [0.1, 0.0], [0.0, 0.1], [0.12, 0.09]])
+        elems = np.array([[0, 1, 2], [2, 1, 3]])
+    s, delta = 0.4, 0.3
+    nV = len(verts)
+    c = oracle.kernel_value("fractional", [0.05, 0.0], [0.0, 0.0], delta ** 2, s)[0] * 0.05 ** (2 + 2 * s)
+    ident = np.zeros((nV, nV))
+    for e in elems:
+        ident[np.ix_(e, e)] += _identical_pair_exact(verts[e], s, c)
+    cross = sum(_cross_direct(verts[elems[i]], list(elems[i]), verts[elems[j]], list(elems[j]), nV, s, c, 32)
+                for i, j in ((0, 1), (1, 0)))
+    want = ident + cross
+    mesh = mesh_dict(elems, np.array([1, 1]), verts, np.ones(nV, dtype=np.int64))
+    conf = make_configuration(close="fractional", tensorGaussDegree=12)
+    kern = {"function": "fractional", "horizon": delta, "outputdim": 1, "fractional_s": s}
+    r = oracle.par_system(mesh, kern, conf)
+    assert r["n_close"] == 4
+    A = r["A"].toarray()
+    scale = np.abs(want).max()
+    err = np.abs(A - want).max() / scale
+    print(f"[fractional] {layout} pair, s=0.4, degree 12: relative error {err:.2e}")
+    assert err < tol
+    assert np.abs(A - (ident - cross)).max() / scale > 100 * tol  # cross pairs with the wrong sign
+    assert np.abs(A - ident).max() / scale > 100 * tol            # cross pairs dropped
