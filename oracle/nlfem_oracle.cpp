@@ -27,11 +27,13 @@
 // against a direct quadrature, to the quadrature error of ~3e-4 (tests/test_oracle_fractional.py),
 // retriangulation by disk-triangle area bounds and hand cases, BFS by brute force.
 //
-// Parity unpinned: integrate_weakSingular at s != -1 (its s = -1 special case and the Duffy
-// tables it uses are checked), the touching pairs of integrate_fractional below the ~3e-4 of
-// their check, and the cut-pair values of the non-polynomial kernels (fractional,
-// microelastic), which share the integration loop that the polynomial kernels pin but whose
-// 7-point sums have no exact value to compare with.
+// Both close-pair routines at s=-0.5: identical pair to 1e-12 of the semi-analytic value,
+// vertex-sharing pair to 1e-10 and edge-sharing pair to 2e-6 of a direct quadrature.
+//
+// Parity unpinned: the close-pair routines' touching pairs at other s below the accuracy of
+// their direct-quadrature checks, and the cut-pair values of the non-polynomial kernels
+// (fractional, microelastic), which share the integration loop that the polynomial kernels
+// pin but whose 7-point sums have no exact value to compare with.
 
 #include <cmath>
 #include <cstdio>

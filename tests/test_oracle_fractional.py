@@ -161,3 +161,39 @@ def test_fractional_touching_pairs_at_s_04(layout, tol):
     assert err < tol
     assert np.abs(A - (ident - cross)).max() / scale > 100 * tol  # cross pairs with the wrong sign
     assert np.abs(A - ident).max() / scale > 100 * tol            # cross pairs dropped
+
+
+@pytest.mark.parametrize("method", ["fractional", "weakSingular"])
+def test_close_pair_routines_at_s_minus_half(method):
+    """At s = -0.5 (|z|^-1) the factor left after the Duffy map is a polynomial, so the tensor
+    Gauss rule converges exponentially and both close-pair routines (integrate_fractional,
+    integrate_weakSingular, which assemble the same form) can be pinned tightly: the identical
+    pair to 1e-12 of the semi-analytic value at degree 12, the vertex-sharing pair to 1e-10 and
+    the edge-sharing pair to 2e-6 of the direct quadrature (whose own error limits the last)."""
+    s, delta = -0.5, 0.3
+    c = oracle.kernel_value("fractional", [0.05, 0.0], [0.0, 0.0], delta ** 2, s)[0] * 0.05 ** (2 + 2 * s)
+    kern = {"function": "fractional", "horizon": delta, "outputdim": 1, "fractional_s": s}
+    conf = make_configuration(close=method, tensorGaussDegree=12)
+    E = np.array([[0.0, 0.0], [0.1, 0.02], [0.03, 0.09]])
+    exact = _identical_pair_exact(E, s, c)
+    mesh = mesh_dict(np.array([[0, 1, 2]]), np.array([1]), E, np.ones(3, dtype=np.int64))
+    A = oracle.par_system(mesh, kern, conf)["A"].toarray()
+    assert np.abs(A - exact).max() < 1e-12 * np.abs(exact).max()
+    for layout, tol in (("vertex", 1e-10), ("edge", 2e-6)):
+        if layout == "vertex":
+            verts = np.array([[0.0, 0.0], [0.1, 0.0], [0.0, 0.1], [-0.1, 0.02], [-0.03, -0.1]])
+            elems = np.array([[0, 1, 2], [0, 3, 4]])
+        else:
+            verts = np.array([[0.0, 0.0], [0.1, 0.0], [0.0, 0.1], [0.12, 0.09]])
+            elems = np.array([[0, 1, 2], [2, 1, 3]])
+        nV = len(verts)
+        want = np.zeros((nV, nV))
+        for e in elems:
+            want[np.ix_(e, e)] += _identical_pair_exact(verts[e], s, c)
+        want += sum(_cross_direct(verts[elems[i]], list(elems[i]), verts[elems[j]], list(elems[j]), nV, s, c, 32)
+                    for i, j in ((0, 1), (1, 0)))
+        m2 = mesh_dict(elems, np.array([1, 1]), verts, np.ones(nV, dtype=np.int64))
+        A = oracle.par_system(m2, kern, conf)["A"].toarray()
+        err = np.abs(A - want).max() / np.abs(want).max()
+        print(f"[close] {method} {layout} pair, s=-0.5: relative error {err:.2e}")
+        assert err < tol, (layout, err)
