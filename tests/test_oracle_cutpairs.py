@@ -69,7 +69,7 @@ def _area2(E):
     return abs((E[1, 0] - E[0, 0]) * (E[2, 1] - E[0, 1]) - (E[2, 0] - E[0, 0]) * (E[1, 1] - E[0, 1]))
 
 
-def _independent_blocks(EA, EB, sqdelta, kname, caps):
+def _independent_blocks(EA, EB, sqdelta, kname, caps, s=-1.0):
     tL, tN, tLP, tNP = (np.zeros((3, 3)) for _ in range(4))
     detA = _area2(EA)
     EAr = np.asarray(EA).reshape(3, 2)
@@ -80,8 +80,8 @@ def _independent_blocks(EA, EB, sqdelta, kname, caps):
             continue
         poly = _polygon(n, tris)
         Y, Wt = _points_weights(poly)
-        kxy = np.array([oracle.kernel_value(kname, x, y, sqdelta)[0] for y in Y])
-        kyx = np.array([oracle.kernel_value(kname, y, x, sqdelta)[0] for y in Y])
+        kxy = np.array([oracle.kernel_value(kname, x, y, sqdelta, s)[0] for y in Y])
+        kyx = np.array([oracle.kernel_value(kname, y, x, sqdelta, s)[0] for y in Y])
         psiY = _basis(EB, Y)
         psix = np.array([1 - P7[k, 0] - P7[k, 1], P7[k, 0], P7[k, 1]])
         wk = W7[k] * detA
@@ -167,3 +167,40 @@ def test_independent_quadrature_detects_planted_mistakes():
     assert max(np.abs(b - g).max() for b, g in zip(bad, blocks)) > 1e-6 * np.abs(ref[1]).max()
     # termLocal and termLocalPrime swapped
     assert np.abs(blocks[0] - ref[2]).max() > 1e-6 * np.abs(ref[0]).max()
+
+
+def _separation(EA, EB):
+    A = np.asarray(EA).reshape(3, 2)
+    B = np.asarray(EB).reshape(3, 2)
+    return min(np.linalg.norm(a - b) for a in A for b in B)
+
+
+@pytest.mark.parametrize("kname,s", [("fractional", 0.4), ("linearPrototypeMicroelastic", -0.5)])
+def test_cut_pairs_of_non_polynomial_kernels_consistent(kname, s):
+    """For the non-polynomial kernels the reference's 7-point fan sums are not exact integrals,
+    so they have no exact value to be pinned against at 1e-12.  On cut pairs whose elements lie
+    at least two mesh sizes apart (the kernel is smooth on the scale of the triangles) they must
+    still agree with the independent high-order polygon quadrature to the 7-point rule's
+    truncation error: a kernel evaluated at the wrong points, or with the wrong exponent, fails."""
+    delta, n = 0.2, 12
+    mesh = unit_square(n, delta, jitter=0.2, seed=11)
+    el, vt = np.asarray(mesh["elements"]), np.asarray(mesh["vertices"])
+    pairs = [(a, b) for a, b in _cut_pairs(mesh, delta, 60, seed=5)
+             if _separation(vt[el[a]], vt[el[b]]) >= 2.0 / n][:12]
+    assert len(pairs) >= 8
+    worst = 0.0
+    for aT, bT in pairs:
+        r, got = oracle.integrate_pair("retriangulate", el, vt, aT, bT, P7, W7, P7, W7, delta * delta, kname,
+                                       fractional_s=s)
+        ref = _independent_blocks(vt[el[aT]].ravel(), vt[el[bT]].ravel(), delta * delta, kname, 1, s)
+        scale = max(np.abs(b).max() for b in ref)
+        worst = max(worst, max(np.abs(g - w).max() / scale for g, w in zip(got, ref)))
+    print(f"[cut] {kname}: separated cut pairs, 7-point sums vs high-order quadrature {worst:.2e}")
+    assert worst < 1e-4
+    # the exponent of the fractional kernel mistaken (2 + s instead of 2 + 2s) is far outside
+    if kname == "fractional":
+        aT, bT = pairs[0]
+        _, got = oracle.integrate_pair("retriangulate", el, vt, aT, bT, P7, W7, P7, W7, delta * delta, kname,
+                                       fractional_s=s)
+        bad = _independent_blocks(vt[el[aT]].ravel(), vt[el[bT]].ravel(), delta * delta, kname, 1, 0.2)
+        assert max(np.abs(g - w).max() for g, w in zip(got, bad)) > 1e-2 * max(np.abs(b).max() for b in bad)
