@@ -971,7 +971,7 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
             const int v = hf[i], f = v & 0xffff, k = (unsigned)v >> 16;
             const int rf = f ? atomicAdd(&tfill[b], f) : 0;
             const int rb = k ? atomicAdd(&tfillB[b], k) : 0;
-            hf[i] = (rf & 0xffff) | (rb << 16);
+            hf[i] = (int)(((unsigned)rf & 0xffffu) | ((unsigned)rb << 16));
         } else {
             if (hf[i]) hf[i] = atomicAdd(&tfill[b], hf[i]);
             if (hb[i]) hb[i] = atomicAdd(&tfillB[b], hb[i]);
