@@ -2523,8 +2523,10 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const doubl
 
 // MODE 0 assembles; MODE 1 writes training samples of the surrogate (partial items, exact
 // FP64 inner sums as targets) and assembles nothing.
+// The FP64 outdim-2 instances fit 168 registers without spilling (3 wavefronts per SIMD where
+// the chunk's LDS allows it: 64 items per chunk) instead of taking 192.
 template <int KER, int OD, typename T, int MODE = 0, int ITEMS = CUT_ITEMS>
-__global__ __launch_bounds__(256) void k_cut(NumArgs a) {
+__global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? 3 : 1) void k_cut(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
     static_assert(ITEMS == 64 || ITEMS == 128 || ITEMS == 256, "items per chunk: 64, 128 or 256");
     constexpr int IW = ITEMS / 64;  // wavefronts that hold an item each
