@@ -1079,9 +1079,12 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
     const int base = BM ? (a.is_DG ? (int)t0 : a.vel[t0]) - a.whalf : 0;
     auto einsert = [&](int U) {
         if constexpr (BM) {
+            // most inserts repeat an element (the six elements of a row share most of their
+            // lists): a plain read first, which lanes on one word share, and the atomic only
+            // for a bit not yet set
             const int idx = U - base;
             if ((unsigned)idx >= (unsigned)(32 * a.EH)) ovf = 8;
-            else atomicOr((unsigned*)&eh[idx >> 5], 1u << (idx & 31));
+            else if (!(((unsigned)eh[idx >> 5] >> (idx & 31)) & 1u)) atomicOr((unsigned*)&eh[idx >> 5], 1u << (idx & 31));
         } else {
             if (lds_insert(eh, a.EH, U) < 0) ovf = 1;
         }
