@@ -892,11 +892,17 @@ NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 -
 // one global atomic per distinct b (with a bounded probe; keys that find no slot use global
 // atomics per entry).
 constexpr int TH = 4096;
-NLFEM_DEV int lds_slot_b(int* h, int key) {  // insert with at most 64 probes; -1 if none free
+// Insert with at most 64 probes; -1 if none free.  A key recurs up to tg times per block, so
+// each probe reads the slot first and takes the compare-and-swap only on an empty slot.
+NLFEM_DEV int lds_slot_b(int* h, int key) {
     uint32_t s = ((uint32_t)key * 2654435761u) & (uint32_t)(TH - 1);
     for (int p = 0; p < 64; p++) {
-        int old = atomicCAS(&h[s], -1, key);
-        if (old == -1 || old == key) return (int)s;
+        const int cur = h[s];
+        if (cur == key) return (int)s;
+        if (cur == -1) {
+            const int old = atomicCAS(&h[s], -1, key);
+            if (old == -1 || old == key) return (int)s;
+        }
         s = (s + 1) & (uint32_t)(TH - 1);
     }
     return -1;
