@@ -1906,12 +1906,16 @@ NLFEM_DEV ColMap colmap_build(int* h, int H, const int* col, const long long* rp
         }
         if (tid == 0 && tot > H) atomicOr(&counters[CNT_ERROR], ERR_HASH);
     } else {
-        for (int i = tid; i < H; i += 256) h[i] = -1;
+        // the hash uses the largest power of two within the H slots (H need not be one: the
+        // rank mode sizes the table by the rows, not by a power of two)
+        const int Hh = 1 << (31 - __clz(H));
+        cm.H = Hh;
+        for (int i = tid; i < Hh; i += 256) h[i] = -1;
         __syncthreads();
         for (int j = 0; j < 3; j++)
             if (use[j])
                 for (int i = tid; i < rlen[j]; i += 256)
-                    if (lds_slot(h, H, col[rp[j] + i]) < 0) atomicOr(&counters[CNT_ERROR], ERR_HASH);
+                    if (lds_slot(h, Hh, col[rp[j] + i]) < 0) atomicOr(&counters[CNT_ERROR], ERR_HASH);
     }
     __syncthreads();
     return cm;
@@ -3967,7 +3971,10 @@ void plan_assemble(Plan& P, const Estimates& est) {
             P.ptotal = ptotal * P.oo;
         }
         na.tptr = P.d_tptr; na.tadj = P.d_tadj; na.tF = P.d_tF; na.poff = P.d_poff; na.P = P.d_P;
-        long long hn = next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+        // accumulator slots: 1.3x the longest row, in multiples of 64 (the column map's rank mode
+        // needs one slot per distinct node of the element's three rows; an overflow is reported
+        // and the table doubled)
+        long long hn = ((long long)(1.3 * std::max(P.maxRow, 1)) + 63) / 64 * 64;
         for (int attempt = 0;; attempt++) {
             NumArgs ni = na;
             ni.HN = (int)hn;
