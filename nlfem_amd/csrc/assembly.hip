@@ -2300,20 +2300,38 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
     const long long len = (rowptr[r + 1] - r0) * oo;
     __shared__ long long base[MAX_DUAL + 2];
     __shared__ int nb;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        // the node's elements with a nonzero label, in incidence order: one lane per element
+        // (their loads overlap), positions from ballot ranks
+        const int lane = threadIdx.x;
         int n = 0;
         if (is_DG) {
             const int T = r / 3;
-            if (elem[T].label != 0) base[n++] = poff[r] * oo;
+            if (elem[T].label != 0) {
+                if (lane == 0) base[0] = poff[r] * oo;
+                n = 1;
+            }
         } else {
-            for (long long q = vptr[r]; q < vptr[r + 1] && n < MAX_DUAL + 2; q++) {
-                const int T = vel[q];
-                if (elem[T].label == 0) continue;
-                const int li = tri[3 * T] == r ? 0 : (tri[3 * T + 1] == r ? 1 : 2);
-                base[n++] = poff[3 * T + li] * oo;
+            const long long q1 = vptr[r + 1];
+            for (long long qb = vptr[r]; qb < q1 && n < MAX_DUAL + 2; qb += 64) {
+                const long long q = qb + lane;
+                bool ok = false;
+                long long b = 0;
+                if (q < q1) {
+                    const int T = vel[q];
+                    if (elem[T].label != 0) {
+                        const int li = tri[3 * T] == r ? 0 : (tri[3 * T + 1] == r ? 1 : 2);
+                        b = poff[3 * T + li] * oo;
+                        ok = true;
+                    }
+                }
+                const unsigned long long m = __ballot(ok);
+                const int rk = n + __popcll(m & ((1ull << lane) - 1ull));
+                if (ok && rk < MAX_DUAL + 2) base[rk] = b;
+                n = min(n + __popcll(m), MAX_DUAL + 2);
             }
         }
-        nb = n;
+        if (lane == 0) nb = n;
     }
     __syncthreads();
     for (long long p = threadIdx.x; p < len; p += 256) {
