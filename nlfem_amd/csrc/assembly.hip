@@ -2509,10 +2509,13 @@ struct ConstantItem {
 };
 
 // wa[3 k + i] = |aT| dx_k psi_i(x_k) and wd[k] = dx_k |aT| are the block's outer weights in LDS.
-template <int OD>
+// KSYM: the kernel matrix is symmetric (k01 = k10, the microelastic field kernel), so phase B
+// accumulates component 1 only and component 2 is read from component 1.
+template <int OD, bool KSYM = false>
 NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const double* wd, const double* iacc,
                          double* pacc, double* tls) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    auto so = [](int o) { return (KSYM && o == 2) ? 1 : o; };
     const int nPx = a.q.nPx;
     const int nw = np * 2 * NB;
     for (int t = threadIdx.x; t < nw + nPx * OO; t += 256) {
@@ -2523,7 +2526,7 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const doubl
             double v = 0;
             if (e < NB) {
                 const int ra = e / N, cb = e - ra * N;
-                const int ai = ra / OD, src = OO + (ra % OD) * OD + OO * (cb / OD) + (cb % OD);
+                const int ai = ra / OD, src = OO + so((ra % OD) * OD + (cb % OD)) + OO * (cb / OD);
 #pragma unroll 4
                 for (int k = 0; k < nPx; k++) {
 #pragma clang fp contract(fast)
@@ -2534,7 +2537,7 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const doubl
                 const int aa = f / (3 * OO), r1 = f - aa * 3 * OO;
                 const int oq = r1 / (3 * OD), r2 = r1 - oq * 3 * OD;
                 const int bb = r2 / OD, o = oq * OD + (r2 - bb * OD);
-                const int src = 4 * OO + sym6(aa, bb) * OO + o;
+                const int src = 4 * OO + sym6(aa, bb) * OO + so(o);
 #pragma unroll 4
                 for (int k = 0; k < nPx; k++) {
 #pragma clang fp contract(fast)
@@ -2546,7 +2549,7 @@ NLFEM_DEV void cut_outer(const NumArgs& a, int np, const double* wa, const doubl
             // innerLocal summed over the chunk's pairs per (outer point, component)
             const int k = (t - nw) / OO, o = (t - nw) % OO;
             double v = 0;
-            for (int p = 0; p < np; p++) v += iacc[((size_t)p * nPx + k) * 10 * OO + o];
+            for (int p = 0; p < np; p++) v += iacc[((size_t)p * nPx + k) * 10 * OO + so(o)];
             tls[k * OO + o] += v;
         }
     }
@@ -2703,12 +2706,17 @@ __global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? 3 : 1) void k_cu
                 inner_point<OD, T>(K, (T)(a.q.dy[i] * rTdet), psi, inL, inN, sLP);
             }
             double* ia = iacc + it * 10 * OO;  // the ~2.5 triangles of an item share these slots
+            // a symmetric kernel matrix: component 2 equals component 1 and is not summed
+            constexpr bool KSYM = KER == KER_MICRO_FIELD;
 #pragma unroll
-            for (int o = 0; o < OO; o++) atomicAdd(&ia[o], inL[o]);
+            for (int o = 0; o < OO; o++)
+                if (!(KSYM && o == 2)) atomicAdd(&ia[o], inL[o]);
 #pragma unroll
-            for (int o = 0; o < 3 * OO; o++) atomicAdd(&ia[OO + o], inN[o]);
+            for (int o = 0; o < 3 * OO; o++)
+                if (!(KSYM && o % OO == 2)) atomicAdd(&ia[OO + o], inN[o]);
 #pragma unroll
-            for (int o = 0; o < 6 * OO; o++) atomicAdd(&ia[4 * OO + o], sLP[o]);
+            for (int o = 0; o < 6 * OO; o++)
+                if (!(KSYM && o % OO == 2)) atomicAdd(&ia[4 * OO + o], sLP[o]);
         }
         __syncthreads();
         if constexpr (MODE == 1) {  // training samples: ratios of the exact sums to the full-triangle sums
@@ -2738,7 +2746,7 @@ __global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? 3 : 1) void k_cu
         }
         // C: outer accumulation (src/integration.cpp:508-537), one thread per (pair, entry) of
         // termNonloc, termLocalPrime and termLocal; termLocal is summed over the block in LDS
-        cut_outer<OD>(a, np, wa, wd, iacc, pacc, tls);
+        cut_outer<OD, KER == KER_MICRO_FIELD>(a, np, wa, wd, iacc, pacc, tls);
         __syncthreads();
         // D: scatter (src/Cassemble.cpp:721-770)
         cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
