@@ -1460,7 +1460,9 @@ struct PairBlocks {
 // Follows src/integration.cpp:424-476.
 // sLP holds the six symmetric products psi_a psi_b (a <= b) times k w, summed over the inner
 // points of one outer point; add_lp scales them by dx_k |aT| into termLocalPrime.
-template <int OD, typename T, bool WL = true, bool WN = true, bool WLP = true>
+// KSYM (a symmetric kernel matrix, OD = 2): component 2 equals component 1 and is not summed
+// into inL / inN (the caller reads component 1 in its place).
+template <int OD, typename T, bool WL = true, bool WN = true, bool WLP = true, bool KSYM = false>
 NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double* inL, double* inN, double* sLP) {
 #pragma clang fp contract(fast)
     constexpr int OO = OD * OD;
@@ -1468,13 +1470,14 @@ NLFEM_DEV void inner_point(const T* K, T w, const T* psi, double* inL, double* i
 #pragma unroll
     for (int o = 0; o < OO; o++) {
         kw[o] = K[o] * w;
-        if (WL) inL[o] += (double)kw[o];
+        if (WL && !(KSYM && o == 2)) inL[o] += (double)kw[o];
     }
     if (WN) {
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
-            for (int o = 0; o < OO; o++) inN[b * OO + o] += (double)(psi[b] * kw[o]);
+            for (int o = 0; o < OO; o++)
+                if (!(KSYM && o == 2)) inN[b * OO + o] += (double)(psi[b] * kw[o]);
     }
     if (WLP) {
         const T pp[6] = {psi[0] * psi[0], psi[0] * psi[1], psi[0] * psi[2], psi[1] * psi[1], psi[1] * psi[2],
@@ -1967,6 +1970,14 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         }
         if (tid < NPY) toPhys(A.E, a.q.Py[2 * tid], a.q.Py[2 * tid + 1], &yi[2 * tid]);
         __syncthreads();
+        // the field kernel's matrix is symmetric: component 2 (k10) equals component 1 (k01), so
+        // it is not summed (KS), and every block entry of component 2 reads its component-1 twin
+        constexpr bool KS = KER == KER_MICRO_FIELD && OD == 2;
+        auto so = [](int o) { return (KS && o == 2) ? 1 : o; };
+        // position in tN of entry (row T_i comp ao, col U_j comp bo), component 2 -> its twin
+        auto tni = [](int i, int ao, int j, int bo) {
+            return (KS && ao == 1 && bo == 0) ? N * (i * OD) + j * OD + 1 : N * (i * OD + ao) + j * OD + bo;
+        };
         // T as outer element (pair_interior with termLocal, without termLocalPrime).  termLocal
         // depends on (i, j) through psi_i psi_j only: six symmetric sums per component.
         double tL6[6 * OO];
@@ -1996,19 +2007,21 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     T K[OO];
                     kernel_eval<KER, OO, T>((T)(x0 - yu[2 * l]), (T)(x1 - yu[2 * l + 1]), a.kp, K);
                     const T psi[3] = {(T)a.q.psiy[3 * l], (T)a.q.psiy[3 * l + 1], (T)a.q.psiy[3 * l + 2]};
-                    inner_point<OD, T, true, true, false>(K, (T)(a.q.dy[l] * B.absDet), psi, inL, inN, nullptr);
+                    inner_point<OD, T, true, true, false, KS>(K, (T)(a.q.dy[l] * B.absDet), psi, inL, inN, nullptr);
                 }
                 {
 #pragma clang fp contract(fast)
 #pragma unroll
                     for (int q = 0; q < 6; q++)
 #pragma unroll
-                        for (int o = 0; o < OO; o++) tL6[q * OO + o] += wl[6 * k + q] * inL[o];
+                        for (int o = 0; o < OO; o++)
+                            if (!(KS && o == 2)) tL6[q * OO + o] += wl[6 * k + q] * inL[o];
 #pragma unroll
                     for (int r = 0; r < N; r++)
 #pragma unroll
                         for (int c = 0; c < N; c++)
-                            tN[N * r + c] += wxa[3 * k + r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
+                            if (!(KS && r % OD == 1 && c % OD == 0))
+                                tN[N * r + c] += wxa[3 * k + r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
                 }
             }
 #pragma unroll
@@ -2022,12 +2035,13 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     for (int ao = 0; ao < OD; ao++)
 #pragma unroll
                         for (int bo = 0; bo < OD; bo++)
-                            atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], -tN[N * (i * OD + ao) + j * OD + bo]);
+                            atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], -tN[tni(i, ao, j, bo)]);
                 }
             }
         }
 #pragma unroll
         for (int t = 0; t < 6 * OO; t++) {
+            if (KS && t % OO == 2) continue;
             double v = tL6[t];
             for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
             tL6[t] = v;
@@ -2036,7 +2050,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
 #pragma unroll
             for (int r = 0; r < N; r++)
 #pragma unroll
-                for (int c = 0; c < N; c++) red[w][N * r + c] = tL6[sym6(r / OD, c / OD) * OO + OD * (r % OD) + (c % OD)];
+                for (int c = 0; c < N; c++) red[w][N * r + c] = tL6[sym6(r / OD, c / OD) * OO + so(OD * (r % OD) + (c % OD))];
         }
         // T as inner element: pairs (U, T).  termLocalPrime of T sums, over the pairs and the
         // outer points x_k of U, dx_k |U| psi_a psi_b(y_l) dy_l |T| k(x_k, y_l); the factor
@@ -2066,11 +2080,12 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     T K[OO];
                     kernel_eval<KER, OO, T>((T)(x[0] - yi[2 * l]), (T)(x[1] - yi[2 * l + 1]), a.kp, K);
                     const T psi[3] = {(T)a.q.psiy[3 * l], (T)a.q.psiy[3 * l + 1], (T)a.q.psiy[3 * l + 2]};
-                    inner_point<OD, T, false, true, false>(K, (T)(a.q.dy[l] * A.absDet), psi, nullptr, inN, nullptr);
+                    inner_point<OD, T, false, true, false, KS>(K, (T)(a.q.dy[l] * A.absDet), psi, nullptr, inN, nullptr);
                     {
 #pragma clang fp contract(fast)
 #pragma unroll
-                        for (int o = 0; o < OO; o++) M[l * OO + o] += wk * (double)K[o];
+                        for (int o = 0; o < OO; o++)
+                            if (!(KS && o == 2)) M[l * OO + o] += wk * (double)K[o];
                     }
                 }
                 {
@@ -2083,7 +2098,8 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     for (int r = 0; r < N; r++)
 #pragma unroll
                         for (int c = 0; c < N; c++)
-                            tN[N * r + c] += wi[r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
+                            if (!(KS && r % OD == 1 && c % OD == 0))
+                                tN[N * r + c] += wi[r / OD] * inN[(r % OD) * OD + OO * (c / OD) + (c % OD)];
                 }
             }
 #pragma unroll
@@ -2097,12 +2113,13 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     for (int ao = 0; ao < OD; ao++)
 #pragma unroll
                         for (int bo = 0; bo < OD; bo++)
-                            atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -tN[N * (i * OD + ao) + j * OD + bo]);
+                            atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -tN[tni(i, ao, j, bo)]);
                 }
             }
         }
 #pragma unroll
         for (int t = 0; t < NPY * OO; t++) {
+            if (KS && t % OO == 2) continue;
             double v = M[t];
             for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
             M[t] = v;
@@ -2118,7 +2135,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                         double v = 0;
 #pragma unroll
                         for (int l = 0; l < NPY; l++)
-                            v += a.q.psiy[3 * l + ia] * a.q.psiy[3 * l + ib] * (a.q.dy[l] * A.absDet) * M[l * OO + o];
+                            v += a.q.psiy[3 * l + ia] * a.q.psiy[3 * l + ib] * (a.q.dy[l] * A.absDet) * M[l * OO + so(o)];
                         red[w][NB + ia * OO * 3 + (o / OD) * OD * 3 + ib * OD + o % OD] = v;
                     }
         }
