@@ -2371,7 +2371,6 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
 //      (registers) and termNonloc of the pair (LDS).
 //   D  scatter of the chunk's pairs, one thread per (pair, i, j).
 constexpr int CUT_ITEMS = 128;    // default items per chunk
-constexpr int CUT_ITEMS_MAX = 256;
 constexpr int CUT_MAXP = 9;
 constexpr int CUT_MAXPAIRS = 32;  // pairs per chunk; needs nPx >= ITEMS / CUT_MAXPAIRS
 // Dynamic LDS of k_cut: polygons (shared with the pair sums), per-item sums, the outer rule's
