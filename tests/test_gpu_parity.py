@@ -239,3 +239,21 @@ def test_cut_items_per_chunk_and_autotune(name):
             asm.set_cut_items(100)
     finally:
         asm.close()
+
+
+@pytest.mark.gpu
+def test_parity_odd_element_count():
+    """An odd number of elements: the BFS with two outer elements per wavefront has an idle
+    second half in its last wavefront.  The square mesh minus one interior element (a hole)
+    gives 199 elements; lists, pattern and values match the oracle, on the bitmap BFS and on
+    the one-element-per-wavefront kernel."""
+    from nlfem_amd.mesh import mesh_dict, unit_square
+    from nlfem_amd.config import make_configuration
+    m = unit_square(10, 0.2, jitter=0.1, seed=3)
+    keep = np.ones(m["nE"], dtype=bool)
+    keep[105] = False
+    odd = mesh_dict(m["elements"][keep], m["elementLabels"][keep], m["vertices"], m["vertexLabels"])
+    assert odd["nE"] % 2 == 1
+    kern = {"function": "constant", "horizon": 0.2, "outputdim": 1}
+    _check_case("odd", odd, kern, make_configuration())
+    _check_case("odd-wave", odd, kern, make_configuration(), paths=nlfem.Assembler.PATH_BFS_WAVE)
