@@ -1290,6 +1290,39 @@ NLFEM_DEV long long find_pos(const long long* rowptr, const int* col, int r, int
     return -1;
 }
 
+// Position of column c in node row r, searched from a guessed rank g: one probe, then a
+// galloping search towards c and a binary search inside the bracket it finds; -1 if absent.
+NLFEM_DEV long long find_pos_from(const long long* rowptr, const int* col, int r, int c, long long g) {
+    const long long lo = rowptr[r], n = rowptr[r + 1] - lo;
+    if (n <= 0) return -1;
+    g = g < 0 ? 0 : (g >= n ? n - 1 : g);
+    const int v = col[lo + g];
+    if (v == c) return lo + g;
+    long long a, b;  // c lies strictly between positions a and b of the row, -1 <= a < b <= n
+    if (v < c) {
+        a = g;
+        long long step = 1;
+        while (a + step < n && col[lo + a + step] < c) { a += step; step <<= 1; }
+        b = a + step < n ? a + step : n;
+        if (b < n && col[lo + b] == c) return lo + b;
+    } else {
+        b = g;
+        long long step = 1;
+        while (b - step >= 0 && col[lo + b - step] > c) { b -= step; step <<= 1; }
+        a = b - step >= 0 ? b - step : -1;
+        if (a >= 0 && col[lo + a] == c) return lo + a;
+    }
+    long long l = a + 1, h = b - 1;
+    while (l <= h) {
+        const long long m = (l + h) >> 1;
+        const int x = col[lo + m];
+        if (x == c) return lo + m;
+        if (x < c) l = m + 1;
+        else h = m - 1;
+    }
+    return -1;
+}
+
 __global__ void k_row_of(int nNodes, const long long* rowptr, int* rowOf) {
     int r = blockIdx.x;
     for (long long p = rowptr[r] + threadIdx.x; p < rowptr[r + 1]; p += blockDim.x) rowOf[p] = r;
@@ -1304,7 +1337,11 @@ __global__ void k_tpos(long long nnz, const long long* rowptr, const int* col, c
     const int r = rowOf[p], c = col[p];
     if (r > c) return;
     if (r == c) { tpos[p] = (int)p; return; }
-    long long q = find_pos(rowptr, col, c, r);
+    // rows of neighbouring nodes share the neighbourhood's offsets, so the rank of r in row c
+    // mirrors the rank of c in row r: len(c) - 1 - rank_r(c) (exact for interior nodes of a
+    // regular mesh, close otherwise)
+    const long long g = (rowptr[c + 1] - rowptr[c]) - 1 - (p - rowptr[r]);
+    long long q = find_pos_from(rowptr, col, c, r, g);
     if (q < 0) { atomicOr(&counters[CNT_ERROR], 4ull); tpos[p] = -1; return; }
     tpos[p] = (int)q;
     tpos[q] = (int)p;
