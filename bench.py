@@ -191,7 +191,9 @@ def main():
     f_dom = f_cut if dom == "cut" else f_int
     t_dom = ms_phase["ms_" + dom] / 1000.0
     achieved = f_dom / t_dom / 1e12
-    roof = {"bound": "mfma", "compute": "fp64 (VALU; FP64 vector and matrix peaks coincide on MI355X)",
+    # the pair quadrature is FP64 vector (VALU) arithmetic, no MFMA: bound by the FP64 vector
+    # peak, which on MI355X equals the FP64 matrix peak
+    roof = {"bound": "fp64-valu", "compute": "fp64 (VALU; FP64 vector and matrix peaks coincide on MI355X)",
             "kernel": cut_kernel_name(conf, kernel) if dom == "cut" else "k_interior",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),

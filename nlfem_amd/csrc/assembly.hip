@@ -224,41 +224,39 @@ __global__ void k_sort_vinc(int nV, const long long* vptr, int* vel) {  // sort 
     }
 }
 
-constexpr int MAX_DUAL = 128;
 // Dual graph row of element e: elements sharing >= 1 vertex, self excluded, ascending
-// (replaces METIS_MeshToDual(ncommon=1), src/Cassemble.cpp:389-417).
+// (replaces METIS_MeshToDual(ncommon=1), src/Cassemble.cpp:389-417).  The row is the union
+// of the three vertices' incidence rows, which are sorted, so a three-way merge yields it in
+// order without a buffer: rows of any length (one artificial vertex shared by hundreds of
+// label-0 elements, as in the reference's tutorial meshes) need no capacity.
 template <bool FILL>
 __global__ void k_dual(int nE, const int* tri, const long long* vptr, const int* vel, int* acnt,
                        const long long* xadj, int* adj, unsigned long long* counters) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nE) return;
-    int buf[MAX_DUAL];
-    int n = 0;
-    bool over = false;
+    long long p[3], q[3];
     for (int j = 0; j < 3; j++) {
-        int v = tri[3 * e + j];
-        for (long long p = vptr[v]; p < vptr[v + 1]; p++) {
-            int f = vel[p];
-            if (f == e) continue;
-            // insert f into the sorted buffer if absent
-            int i = n - 1;
-            bool dup = false;
-            for (int q = 0; q < n; q++)
-                if (buf[q] == f) { dup = true; break; }
-            if (dup) continue;
-            if (n >= MAX_DUAL) { over = true; continue; }
-            while (i >= 0 && buf[i] > f) { buf[i + 1] = buf[i]; i--; }
-            buf[i + 1] = f;
-            n++;
-        }
+        const int v = tri[3 * e + j];
+        p[j] = vptr[v];
+        q[j] = vptr[v + 1];
     }
-    if (over) atomicOr(&counters[CNT_ERROR], 2ull);
+    long long o = FILL ? xadj[e] : 0;
+    int n = 0, last = -1;
+    for (;;) {
+        int f = INT_MAX;
+        for (int j = 0; j < 3; j++)
+            if (p[j] < q[j]) f = min(f, vel[p[j]]);
+        if (f == INT_MAX) break;
+        for (int j = 0; j < 3; j++)
+            if (p[j] < q[j] && vel[p[j]] == f) p[j]++;
+        if (f == e || f == last) continue;  // self; an element listed twice at one vertex
+        last = f;
+        if (FILL) adj[o + n] = f;
+        n++;
+    }
     if (!FILL) {
         acnt[e] = n;
         atomicMax(&counters[CNT_MAXT], (unsigned long long)n);
-    } else {
-        long long o = xadj[e];
-        for (int q = 0; q < n; q++) adj[o + q] = buf[q];
     }
 }
 
@@ -2343,7 +2341,11 @@ __global__ void k_plen(int nE, const ElemRec* elem, const int* tri, const int* v
     }
 }
 
-// Node row r = sum of the partial rows of the elements holding node r.
+// Node row r = sum of the partial rows of the elements holding node r.  The node's incidence
+// row is taken in chunks of MERGE_CHUNK elements (one chunk on regular meshes; a vertex shared
+// by more elements, such as an artificial vertex joining label-0 elements, takes several, the
+// later ones adding to the values the first one stored).
+constexpr int MERGE_CHUNK = 128;
 __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, const ElemRec* elem, const int* tri,
                                                const int* vlabel, const long long* vptr, const int* vel,
                                                const long long* rowptr, const long long* poff, const double* P,
@@ -2352,46 +2354,52 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
     if (!row_labelled(vlabel, r, is_DG)) return;
     const long long r0 = rowptr[r];
     const long long len = (rowptr[r + 1] - r0) * oo;
-    __shared__ long long base[MAX_DUAL + 2];
+    __shared__ long long base[MERGE_CHUNK];
     __shared__ int nb;
-    if (threadIdx.x < 64) {
-        // the node's elements with a nonzero label, in incidence order: one lane per element
-        // (their loads overlap), positions from ballot ranks
-        const int lane = threadIdx.x;
-        int n = 0;
-        if (is_DG) {
-            const int T = r / 3;
-            if (elem[T].label != 0) {
-                if (lane == 0) base[0] = poff[r] * oo;
-                n = 1;
-            }
-        } else {
-            const long long q1 = vptr[r + 1];
-            for (long long qb = vptr[r]; qb < q1 && n < MAX_DUAL + 2; qb += 64) {
-                const long long q = qb + lane;
-                bool ok = false;
-                long long b = 0;
-                if (q < q1) {
-                    const int T = vel[q];
-                    if (elem[T].label != 0) {
-                        const int li = tri[3 * T] == r ? 0 : (tri[3 * T + 1] == r ? 1 : 2);
-                        b = poff[3 * T + li] * oo;
-                        ok = true;
-                    }
+    const long long qs = is_DG ? 0 : vptr[r], qe = is_DG ? 1 : vptr[r + 1];
+    for (long long c0 = qs; c0 < qe; c0 += MERGE_CHUNK) {
+        if (threadIdx.x < 64) {
+            // the chunk's elements with a nonzero label, in incidence order: one lane per
+            // element (their loads overlap), positions from ballot ranks
+            const int lane = threadIdx.x;
+            int n = 0;
+            if (is_DG) {
+                const int T = r / 3;
+                if (elem[T].label != 0) {
+                    if (lane == 0) base[0] = poff[r] * oo;
+                    n = 1;
                 }
-                const unsigned long long m = __ballot(ok);
-                const int rk = n + __popcll(m & ((1ull << lane) - 1ull));
-                if (ok && rk < MAX_DUAL + 2) base[rk] = b;
-                n = min(n + __popcll(m), MAX_DUAL + 2);
+            } else {
+                const long long q1 = min(qe, c0 + MERGE_CHUNK);
+                for (long long qb = c0; qb < q1; qb += 64) {
+                    const long long q = qb + lane;
+                    bool ok = false;
+                    long long b = 0;
+                    if (q < q1) {
+                        const int T = vel[q];
+                        if (elem[T].label != 0) {
+                            const int li = tri[3 * T] == r ? 0 : (tri[3 * T + 1] == r ? 1 : 2);
+                            b = poff[3 * T + li] * oo;
+                            ok = true;
+                        }
+                    }
+                    const unsigned long long m = __ballot(ok);
+                    const int rk = n + __popcll(m & ((1ull << lane) - 1ull));
+                    if (ok) base[rk] = b;
+                    n += __popcll(m);
+                }
             }
+            if (lane == 0) nb = n;
         }
-        if (lane == 0) nb = n;
-    }
-    __syncthreads();
-    for (long long p = threadIdx.x; p < len; p += 256) {
-        double v = 0.0;
-        for (int t = 0; t < nb; t++) v += P[base[t] + p];
-        val[r0 * oo + p] = v;
+        __syncthreads();
+        const bool first = c0 == qs;
+        for (long long p = threadIdx.x; p < len; p += 256) {
+            double v = 0.0;
+            for (int t = 0; t < nb; t++) v += P[base[t] + p];
+            if (first) val[r0 * oo + p] = v;
+            else val[r0 * oo + p] += v;
+        }
+        __syncthreads();
     }
 }
 
@@ -3857,7 +3865,6 @@ void plan_assemble(Plan& P, const Estimates& est) {
         HIPCHECK(hipGetLastError());
         ev_end(P, PH_BFS);
         read_counters(P, h);
-        if (h[CNT_ERROR]) throw std::runtime_error("mesh error (dual graph row longer than 128 elements)");
         if (!h[CNT_OVERFLOW]) {
             P.bfs_visit = visit; P.bfs_hvis = hvis; P.bfs_fvis = fvis;
             break;
@@ -4088,7 +4095,11 @@ void plan_assemble(Plan& P, const Estimates& est) {
             read_counters(P, h);
             if (!(h[CNT_ERROR] & ERR_HASH)) break;
             if (attempt >= 2) throw std::length_error("interior LDS column table overflow");
-            HIPCHECK(hipMemsetAsync(P.d_counters + CNT_ERROR, 0, sizeof(unsigned long long), st));
+            // clear only the overflow bit: an inconsistency reported earlier (a transposed
+            // position missing) must still reach the final check
+            const unsigned long long keep = h[CNT_ERROR] & ~ERR_HASH;
+            HIPCHECK(hipMemcpyAsync(P.d_counters + CNT_ERROR, &keep, sizeof(keep), hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
             hn *= 2;
         }
         k_merge<<<P.nNodes, 256, 0, st>>>(P.nNodes, P.oo, c.is_DG, P.d_elem, P.d_tri, P.d_vlabel, P.d_vptr, P.d_vel,

@@ -164,6 +164,14 @@ struct Conf {
     bool fullCC = false;
 };
 
+// Planted mistakes, for tests that must show they catch them (0 in normal use):
+// PLANT_BLOCKED_LAYOUT numbers the outdim-2 degrees of freedom component-major
+// (comp * nV + node) instead of the reference's interleaved od * node + comp;
+// PLANT_FIELD_DIAGONAL exchanges k00 and k11 of the field kernel.
+constexpr int PLANT_BLOCKED_LAYOUT = 1, PLANT_FIELD_DIAGONAL = 2;
+static int g_plant = 0;
+extern "C" void oracle_set_plant(int mode) { g_plant = mode; }
+
 // ---------------------------------------------------------------- kernels (src/model.cpp)
 // src/model.cpp:25-28
 void kernel_constant(const double*, long, const double*, long, const Mesh& m, double* k) {
@@ -202,6 +210,7 @@ void kernelField_linearPrototypeMicroelastic(const double* x, long, const double
     k[1] = c * den * z[0] * z[1];
     k[2] = c * den * z[1] * z[0];
     k[3] = c * den * z[1] * z[1];
+    if (g_plant & PLANT_FIELD_DIAGONAL) std::swap(k[0], k[3]);
 }
 
 // forcing: src/model.cpp:240-242 f_linear, :201-203 f_constant, :204-210 fField_linear
@@ -778,20 +787,25 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                 if (first) closec++;
                                 if (keep_neighbours) { nbrs_t[aTdx].push_back(bTdx); close_t[aTdx].push_back(first ? 1 : 0); }
                                 queue.push(bTdx);
-                                // src/Cassemble.cpp:721-770 scatter
+                                // src/Cassemble.cpp:721-770 scatter; degree of freedom (node, component) is
+                                // od * node + component (interleaved, src/Cassemble.cpp:736-766)
+                                auto dof = [&](long node, int comp) -> unsigned long {
+                                    if (g_plant & PLANT_BLOCKED_LAYOUT) return (unsigned long)comp * (K / od) + node;
+                                    return (unsigned long)(od * node + comp);
+                                };
                                 for (int a = 0; a < dV * od; a++) {
                                     for (int b = 0; b < dV * od; b++) {
                                         unsigned long Adx;
                                         if (is_DG || vertLabel[aAdx[a / od]] > 0) {
-                                            Adx = (unsigned long)(od * aAdx[a / od] + a % od) * K + od * aAdx[b / od] + b % od;
+                                            Adx = dof(aAdx[a / od], a % od) * K + dof(aAdx[b / od], b % od);
                                             Ad[Adx] += g_abs ? std::fabs(tL[dV * od * a + b]) : tL[dV * od * a + b];
-                                            Adx = (unsigned long)(od * aAdx[a / od] + a % od) * K + od * bAdx[b / od] + b % od;
+                                            Adx = dof(aAdx[a / od], a % od) * K + dof(bAdx[b / od], b % od);
                                             Ad[Adx] += g_abs ? std::fabs(tN[dV * od * a + b]) : -tN[dV * od * a + b];
                                         }
                                         if (is_DG || vertLabel[bAdx[b / od]] > 0) {
-                                            Adx = (unsigned long)(od * bAdx[b / od] + b % od) * K + od * bAdx[a / od] + a % od;
+                                            Adx = dof(bAdx[b / od], b % od) * K + dof(bAdx[a / od], a % od);
                                             Ad[Adx] += g_abs ? std::fabs(tLP[dV * od * a + b]) : tLP[dV * od * a + b];
-                                            Adx = (unsigned long)(od * bAdx[b / od] + b % od) * K + od * aAdx[a / od] + a % od;
+                                            Adx = dof(bAdx[b / od], b % od) * K + dof(aAdx[a / od], a % od);
                                             Ad[Adx] += g_abs ? std::fabs(tNP[dV * od * a + b]) : -tNP[dV * od * a + b];
                                         }
                                     }

@@ -43,6 +43,7 @@ def load():
     lib.oracle_result_neighbours.argtypes = [C.c_void_p, _lp, _ip, C.POINTER(C.c_ubyte)]
     lib.oracle_result_free.argtypes = [C.c_void_p]
     lib.oracle_set_abs_mode.argtypes = [C.c_int]
+    lib.oracle_set_plant.argtypes = [C.c_int]
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_dual_graph.restype = C.c_long
     lib.oracle_dual_graph.argtypes = [_lp, C.c_int, C.c_int, _ip, _ip]
@@ -124,6 +125,24 @@ def abs_matrix(mesh, kernel, conf):
         return par_system(mesh, kernel, conf)["A"]
     finally:
         lib.oracle_set_abs_mode(0)
+
+
+class planted:
+    """Context manager that plants a known mistake in the oracle (tests that must show they
+    catch it): "blocked_layout" numbers outdim-2 degrees of freedom comp * nV + node instead of
+    2 node + comp; "field_diagonal" exchanges k00 and k11 of the field kernel."""
+    MODES = {"blocked_layout": 1, "field_diagonal": 2}
+
+    def __init__(self, mode):
+        self.mode = self.MODES[mode]
+
+    def __enter__(self):
+        load().oracle_set_plant(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_plant(0)
+        return False
 
 
 def dual_graph(elements, nV):

@@ -73,3 +73,27 @@ def compare(A, B, rtol=1e-12):
 
 def sorted_lists(ptr, nbr):
     return [np.sort(nbr[ptr[i]:ptr[i + 1]]) for i in range(len(ptr) - 1)]
+
+
+def floor_usage(d, addends, tol, floor):
+    """Entries that meet the bar only because of the absolute floor (|d| > tol * sum|addends|),
+    and the largest |d| / sum|addends| among them."""
+    d = np.abs(np.asarray(d))
+    need = d > tol * addends
+    if not np.any(need):
+        return 0, 0.0
+    return int(np.count_nonzero(need)), float(np.max(d[need] / np.maximum(addends[need], 1e-300)))
+
+
+# The floor covers entries whose addends are tiny basis-function products, computed through
+# the constant kernel's factorised (moment) sums: their absolute rounding error is of the
+# scale of the pair's larger entries times the FP64 unit roundoff, i.e. of 1e-16 max|A|.
+# A handful of such entries per matrix may exceed 1e-12 of their own |addend| sum, never
+# by more than the ratio below.
+FLOOR_MAX_FRACTION = 1e-3
+FLOOR_MAX_RATIO = 1e-6
+
+
+def check_floor_usage(name, n_floor, worst_floorless, nnz):
+    assert n_floor <= FLOOR_MAX_FRACTION * nnz, f"{name}: {n_floor} of {nnz} entries rely on the absolute floor"
+    assert worst_floorless <= FLOOR_MAX_RATIO, f"{name}: floorless ratio {worst_floorless:.3e}"

@@ -11,6 +11,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
+from helpers import check_floor_usage, floor_usage
 from nlfem_amd import batch
 from nlfem_amd.config import make_configuration
 from nlfem_amd.mesh import unit_square
@@ -81,8 +82,13 @@ def test_batch_blocks_match_oracle():
         assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
         Abs = oracle.abs_matrix(m, k, conf).tocsr()
         D = (A - R).tocoo()
-        bound = 1e-12 * np.asarray(Abs[D.row, D.col]).ravel() + 1e-15 * abs(R).max()
+        addends = np.asarray(Abs[D.row, D.col]).ravel()
+        floor = 1e-15 * abs(R).max()
+        bound = 1e-12 * addends + floor
         worst = float(np.max(np.abs(D.data) / np.maximum(bound, 1e-300))) if D.nnz else 0.0
-        print(f"[batch] nE={m['nE']} delta={d} worst |diff|/bound = {worst:.3e}", flush=True)
+        n_floor, worst_floorless = floor_usage(D.data, addends, 1e-12, floor)
+        print(f"[batch] nE={m['nE']} delta={d} worst |diff|/bound = {worst:.3e}; entries needing the floor: "
+              f"{n_floor} of {A.nnz} (worst floorless {worst_floorless:.3e})", flush=True)
         assert np.all(np.abs(D.data) <= bound)
+        check_floor_usage(f"batch nE={m['nE']}", n_floor, worst_floorless, A.nnz)
     assert st["pairs"] == total

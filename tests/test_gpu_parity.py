@@ -14,7 +14,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
-from helpers import case, compare, label0_mesh, sorted_lists
+from helpers import case, check_floor_usage, compare, floor_usage, label0_mesh, sorted_lists
 from nlfem_amd import nlfem
 
 pytestmark = pytest.mark.gpu
@@ -66,9 +66,13 @@ def _check_case(name, mesh, kernel, conf, tol=FP64_TOL, paths=0):
     # of the pair scale in the reference's own point-by-point evaluation as well
     floor = 1e-15 * abs(ref["A"]).max()
     bound = tol * addends + floor
+    n_floor, worst_floorless = floor_usage(D.data, addends, tol, floor)
     print(f"[parity] case={name} max |gpu-oracle| / sum|addends| = {worst:.3e}; "
-          f"max |gpu-oracle| / bound = {float(np.max(np.abs(D.data) / bound)) if D.nnz else 0.0:.3e}", flush=True)
+          f"max |gpu-oracle| / bound = {float(np.max(np.abs(D.data) / bound)) if D.nnz else 0.0:.3e}; "
+          f"entries needing the floor: {n_floor} of {A.nnz}, their worst floorless ratio {worst_floorless:.3e}",
+          flush=True)
     assert np.all(np.abs(D.data) <= bound), f"max rel {rel:.3e}"
+    check_floor_usage(name, n_floor, worst_floorless, A.nnz)
     return A, ref
 
 
@@ -101,6 +105,25 @@ def test_mixed_precision_tolerance():
     same, rel, rowrel = compare(A, ref["A"])
     fro = sp.linalg.norm(A - ref["A"]) / sp.linalg.norm(ref["A"])
     print(f"[parity] mixed C2s rel_fro={fro:.3e} max_rowrel={rowrel:.3e}", flush=True)
+    assert same
+    assert fro < 1e-5
+
+
+def test_mixed_precision_fractional_C3s():
+    """Config C3 in its stated precision: fractional kernel s = 0.4, close pairs by tensor Gauss
+    of degree 5 in FP64, far pairs with FP32 quadrature sums (FP32 exp2/log2 of the kernel) and
+    FP64 accumulation, against the FP64 oracle: identical pattern, relative Frobenius < 1e-5."""
+    mesh, kernel, conf = case("C3s")
+    from nlfem_amd.config import make_configuration
+    conf = make_configuration(close="fractional", tensorGaussDegree=5)
+    conf["precision"] = "mixed"
+    ref = oracle.par_system(mesh, kernel, conf)
+    A, _, st = _gpu(mesh, kernel, conf)
+    same, rel, rowrel = compare(A, ref["A"])
+    fro = sp.linalg.norm(A - ref["A"]) / sp.linalg.norm(ref["A"])
+    print(f"[parity] mixed C3s degree 5 close={int(st['close'])} rel_fro={fro:.3e} max_rowrel={rowrel:.3e}",
+          flush=True)
+    assert st["close"] > 0
     assert same
     assert fro < 1e-5
 
@@ -214,9 +237,13 @@ def test_cut_items_per_chunk_and_autotune(name):
     def check(A, label):
         same, rel, _ = compare(A, ref["A"])
         D = (A - ref["A"]).tocoo()
-        bound = FP64_TOL * np.asarray(Abs[D.row, D.col]).ravel() + floor
-        print(f"[parity] case={name} {label} pattern_equal={same} max_rel={rel:.3e}", flush=True)
+        addends = np.asarray(Abs[D.row, D.col]).ravel()
+        bound = FP64_TOL * addends + floor
+        n_floor, worst_floorless = floor_usage(D.data, addends, FP64_TOL, floor)
+        print(f"[parity] case={name} {label} pattern_equal={same} max_rel={rel:.3e} floor entries={n_floor} "
+              f"(worst floorless {worst_floorless:.3e})", flush=True)
         assert same and np.all(np.abs(D.data) <= bound), label
+        check_floor_usage(f"{name} {label}", n_floor, worst_floorless, A.nnz)
 
     for items in (64, 128, 256):
         asm = nlfem.Assembler(mesh, kernel, conf).set_cut_items(items)

@@ -69,8 +69,12 @@ def _area2(E):
     return abs((E[1, 0] - E[0, 0]) * (E[2, 1] - E[0, 1]) - (E[2, 0] - E[0, 0]) * (E[1, 1] - E[0, 1]))
 
 
-def _independent_blocks(EA, EB, sqdelta, kname, caps, s=-1.0):
-    tL, tN, tLP, tNP = (np.zeros((3, 3)) for _ in range(4))
+def _independent_blocks(EA, EB, sqdelta, kname, caps, s=-1.0, od=1):
+    """The four blocks from their definitions.  For outdim 2 the degree of freedom (node i,
+    component c) is row/column 2 i + c, and the kernel's 2 x 2 matrix k_cd multiplies
+    component d of the column into component c of the row."""
+    n3 = 3 * od
+    tL, tN, tLP, tNP = (np.zeros((n3, n3)) for _ in range(4))
     detA = _area2(EA)
     EAr = np.asarray(EA).reshape(3, 2)
     for k in range(P7.shape[0]):
@@ -80,15 +84,25 @@ def _independent_blocks(EA, EB, sqdelta, kname, caps, s=-1.0):
             continue
         poly = _polygon(n, tris)
         Y, Wt = _points_weights(poly)
-        kxy = np.array([oracle.kernel_value(kname, x, y, sqdelta, s)[0] for y in Y])
-        kyx = np.array([oracle.kernel_value(kname, y, x, sqdelta, s)[0] for y in Y])
+        # kernel matrices [npts, od, od] (row-major k00, k01, k10, k11 for outdim 2)
+        kxy = np.array([oracle.kernel_value(kname, x, y, sqdelta, s)[:od * od] for y in Y]).reshape(-1, od, od)
+        kyx = np.array([oracle.kernel_value(kname, y, x, sqdelta, s)[:od * od] for y in Y]).reshape(-1, od, od)
         psiY = _basis(EB, Y)
         psix = np.array([1 - P7[k, 0] - P7[k, 1], P7[k, 0], P7[k, 1]])
         wk = W7[k] * detA
-        tL += wk * np.outer(psix, psix) * np.sum(Wt * kxy)
-        tN += wk * np.outer(psix, psiY.T @ (Wt * kxy))
-        tNP += wk * np.outer(psix, psiY.T @ (Wt * kyx))
-        tLP += wk * (psiY.T * (Wt * kyx)) @ psiY
+        IL = np.einsum("q,qcd->cd", Wt, kxy)             # int_P k(x, y)
+        IN = np.einsum("q,qj,qcd->jcd", Wt, psiY, kxy)    # int_P psi_j(y) k(x, y)
+        INP = np.einsum("q,qj,qcd->jcd", Wt, psiY, kyx)   # int_P psi_j(y) k(y, x)
+        ILP = np.einsum("q,qi,qj,qcd->ijcd", Wt, psiY, psiY, kyx)  # int_P psi_i psi_j k(y, x)
+        for i in range(3):
+            for c in range(od):
+                for j in range(3):
+                    for d in range(od):
+                        r, col = i * od + c, j * od + d
+                        tL[r, col] += wk * psix[i] * psix[j] * IL[c, d]
+                        tN[r, col] += wk * psix[i] * IN[j, c, d]
+                        tNP[r, col] += wk * psix[i] * INP[j, c, d]
+                        tLP[r, col] += wk * ILP[i, j, c, d]
     return tL, tN, tLP, tNP
 
 
@@ -175,7 +189,8 @@ def _separation(EA, EB):
     return min(np.linalg.norm(a - b) for a in A for b in B)
 
 
-@pytest.mark.parametrize("kname,s", [("fractional", 0.4), ("linearPrototypeMicroelastic", -0.5)])
+@pytest.mark.parametrize("kname,s", [("fractional", 0.4), ("linearPrototypeMicroelastic", -0.5),
+                                     ("linearPrototypeMicroelasticField", -0.5)])
 def test_cut_pairs_of_non_polynomial_kernels_consistent(kname, s):
     """For the non-polynomial kernels the reference's 7-point fan sums are not exact integrals,
     so they have no exact value to be pinned against at 1e-12.  On cut pairs whose elements lie
@@ -188,15 +203,30 @@ def test_cut_pairs_of_non_polynomial_kernels_consistent(kname, s):
     pairs = [(a, b) for a, b in _cut_pairs(mesh, delta, 60, seed=5)
              if _separation(vt[el[a]], vt[el[b]]) >= 2.0 / n][:12]
     assert len(pairs) >= 8
+    od = 2 if kname.endswith("Field") else 1
     worst = 0.0
     for aT, bT in pairs:
         r, got = oracle.integrate_pair("retriangulate", el, vt, aT, bT, P7, W7, P7, W7, delta * delta, kname,
-                                       fractional_s=s)
-        ref = _independent_blocks(vt[el[aT]].ravel(), vt[el[bT]].ravel(), delta * delta, kname, 1, s)
+                                       fractional_s=s, outdim=od)
+        ref = _independent_blocks(vt[el[aT]].ravel(), vt[el[bT]].ravel(), delta * delta, kname, 1, s, od)
         scale = max(np.abs(b).max() for b in ref)
         worst = max(worst, max(np.abs(g - w).max() / scale for g, w in zip(got, ref)))
     print(f"[cut] {kname}: separated cut pairs, 7-point sums vs high-order quadrature {worst:.2e}")
     assert worst < 1e-4
+    if od == 2:
+        # component layout mistaken (the 2 x 2 kernel blocks transposed into the node blocks, i.e.
+        # row 3 c + i instead of 2 i + c) and the kernel's diagonal exchanged are far outside
+        aT, bT = pairs[0]
+        _, got = oracle.integrate_pair("retriangulate", el, vt, aT, bT, P7, W7, P7, W7, delta * delta, kname,
+                                       fractional_s=s, outdim=2)
+        ref = _independent_blocks(vt[el[aT]].ravel(), vt[el[bT]].ravel(), delta * delta, kname, 1, s, 2)
+        perm = np.array([0, 2, 4, 1, 3, 5])
+        scale = max(np.abs(b).max() for b in ref)
+        assert max(np.abs(g[np.ix_(perm, perm)] - w).max() for g, w in zip(got, ref)) > 1e-2 * scale
+        with oracle.planted("field_diagonal"):
+            _, bad = oracle.integrate_pair("retriangulate", el, vt, aT, bT, P7, W7, P7, W7, delta * delta, kname,
+                                           fractional_s=s, outdim=2)
+        assert max(np.abs(g - w).max() for g, w in zip(bad, ref)) > 1e-2 * scale
     # the exponent of the fractional kernel mistaken (2 + s instead of 2 + 2s) is far outside
     if kname == "fractional":
         aT, bT = pairs[0]

@@ -1,0 +1,94 @@
+"""The oracle's vector-valued (outdim 2) path against the rigid motions that its operator
+annihilates.
+
+For the field kernel C(x, y) = c z z^T / |z|^3 (z = x - y, src/model.cpp:140-153), the
+operator L u(x) = int (C(x, y) u(x) - C(y, x) u(y)) dy vanishes on every rigid motion u(x) =
+t + w R x (t a translation, R the rotation by 90 degrees): C is even in z, so the integrand is
+C (u(x) - u(y)) = w C R z, and z^T R z = 0.  P1 elements reproduce rigid motions exactly, so the
+discrete statement holds at every pair of quadrature points: each assembled row of A applied to
+the interpolant of a rigid motion is zero to rounding, separately for t_x = (1, 0), t_y = (0, 1)
+and u_rot = (-x1, x0) — provided the degrees of freedom are numbered as the reference numbers
+them, (node, component) -> 2 node + component (src/Cassemble.cpp:736-766).
+
+The two translations alone are blind to a consistent relabelling of (node, component) pairs, and
+to a kernel whose components are mixed up; the rotation is not.  Planted mistakes (a
+component-major numbering, and k00 exchanged with k11) must fail it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import case
+from nlfem_amd.config import make_configuration
+from nlfem_amd.mesh import unit_square
+
+FIELD = {"function": "linearPrototypeMicroelasticField", "outputdim": 2, "fractional_s": -0.5}
+
+
+def _rigid(mesh):
+    """t_x, t_y and u_rot in the reference's interleaved numbering (2 node + component)."""
+    x = np.asarray(mesh["vertices"])
+    nV = x.shape[0]
+    tx, ty, ur = np.zeros(2 * nV), np.zeros(2 * nV), np.zeros(2 * nV)
+    tx[0::2] = 1.0
+    ty[1::2] = 1.0
+    ur[0::2] = -x[:, 1]
+    ur[1::2] = x[:, 0]
+    return {"t_x": tx, "t_y": ty, "u_rot": ur}
+
+
+def _residuals(mesh, kernel, conf):
+    """max over assembled rows of |(A u)_r| / (|A| |u|)_r for each rigid motion u, where |A| is
+    the oracle's matrix of absolute addends (the scale of the rounding an entry can carry)."""
+    A = oracle.par_system(mesh, kernel, conf)["A"].tocsr()
+    Abs = oracle.abs_matrix(mesh, kernel, conf).tocsr()
+    rows = np.where(mesh["nodeLabels"] > 0)[0]
+    out = {}
+    for name, u in _rigid(mesh).items():
+        r = (A @ u)[rows]
+        s = (Abs @ np.abs(u))[rows]
+        out[name] = float(np.max(np.abs(r) / np.maximum(s, 1e-300)))
+    return out
+
+
+def _cases():
+    c5s = case("C5s")
+    # horizon below the mesh size: nearly every pair is cut by the ball boundary
+    k = dict(FIELD)
+    k["horizon"] = 0.08
+    cut = (unit_square(10, 0.08, jitter=0.2, seed=4, outdim=2), k,
+           make_configuration(close="retriangulate", outer="16point", inner="7point"))
+    # without cap points (a different polygon on every cut pair), larger horizon
+    kn = dict(FIELD)
+    kn["horizon"] = 0.2
+    nocaps = (unit_square(8, 0.2, jitter=0.15, seed=6, outdim=2), kn,
+              make_configuration(close="retriangulate", outer="16point", inner="7point", caps=0))
+    return {"C5s": c5s, "cut-dominated": cut, "nocaps": nocaps}
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_rigid_motions_in_null_space(name):
+    mesh, kernel, conf = CASES[name]
+    res = _residuals(mesh, kernel, conf)
+    print(f"[field] {name}: {res}")
+    for k, v in res.items():
+        assert v < 1e-12, (name, k, v)
+
+
+def test_planted_layout_and_kernel_mistakes_fail():
+    mesh, kernel, conf = CASES["cut-dominated"]
+    with oracle.planted("blocked_layout"):
+        bad = _residuals(mesh, kernel, conf)
+    print(f"[field] component-major numbering: {bad}")
+    assert bad["u_rot"] > 1e-9  # at least 1e3 times the bar
+    with oracle.planted("field_diagonal"):
+        bad = _residuals(mesh, kernel, conf)
+    print(f"[field] k00 <-> k11: {bad}")
+    # the translations stay in the null space (partition of unity), the rotation does not
+    assert bad["t_x"] < 1e-12 and bad["t_y"] < 1e-12
+    assert bad["u_rot"] > 1e-9  # at least 1e3 times the bar
+    # and the oracle is back to normal afterwards
+    assert max(_residuals(mesh, kernel, conf).values()) < 1e-12
