@@ -154,6 +154,18 @@ int nlfem_load_vector(int K, const long* ptrTriangles, const long* ptrLabelTrian
  * ------------------------------------------------------------------------------------ */
 int nlfem_mlp_info(int* nin, int* nhidden, int* nout, int* nparam, int* nnorm);
 int nlfem_mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y);
+/* Allocation-free variant of nlfem_mlp_forward.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  All pointers are DEVICE memory owned by the caller:
+ * d_params[9737], d_norm[42], d_X[n][12], d_Y[n][9]; stream is a hipStream_t (NULL: the
+ * default stream); the call only enqueues work (no synchronisation).  It calls no hipMalloc,
+ * hipFree or other allocating routine: the packed BF16 network lives in a fixed device array
+ * of the library, 39,744 bytes per device, held until the process ends.  n must lie in
+ * [0, 2^31 - 1] (NLFEM_EINVAL otherwise).  Every call on a device repacks that device's one
+ * copy, so calls that overlap on one device (different streams or threads) race: serialise
+ * them.  Results equal nlfem_mlp_forward's bit for bit (same kernels). */
+int nlfem_mlp_forward_noalloc(const float* d_params, const float* d_norm, const float* d_X, long long n, float* d_Y,
+                              void* stream);
 /* Gradient of the mean squared error in normalised output units over all n samples. */
 int nlfem_mlp_grad(const float* params, const float* norm, const float* X, const float* T, long long n,
                    float* grad, float* loss);

@@ -53,6 +53,8 @@ SIGNATURES = {
                                                _fp, _fp, _llp]),
     "nlfem_mlp_info": (C.c_int, [_ip, _ip, _ip, _ip, _ip]),
     "nlfem_mlp_forward": (C.c_int, [_fp, _fp, _fp, C.c_longlong, _fp]),
+    "nlfem_mlp_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p,
+                                            C.c_void_p]),
     "nlfem_mlp_grad": (C.c_int, [_fp, _fp, _fp, _fp, C.c_longlong, _fp, _fp]),
     "nlfem_mlp_train": (C.c_int, [_fp, _fp, _fp, _fp, C.c_longlong, C.c_int, C.c_int, C.c_float, C.c_uint, _fp]),
 }

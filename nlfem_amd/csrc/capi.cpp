@@ -28,6 +28,8 @@ void run_retriangulate(int n, const double* x, const double* TE, double sqdelta,
 void run_geom(int n, int op, const double* E, const double* p, double* out, int* ok);
 namespace mlp {
 void mlp_forward(const float* params, const float* norm, const float* X, long long n, float* Y);
+void mlp_forward_noalloc(const float* d_params, const float* d_norm, const float* d_X, long long n, float* d_Y,
+                         hipStream_t st);
 void mlp_grad(const float* params, const float* norm, const float* X, const float* T, long long n, float* grad,
               float* loss);
 void mlp_train(float* params, const float* norm, const float* X, const float* T, long long n, int epochs, int batch,
@@ -590,6 +592,15 @@ extern "C" int nlfem_mlp_forward(const float* params, const float* norm, const f
     if (!params || !norm || (n > 0 && (!X || !Y)) || n < 0) return fail(NLFEM_EINVAL, "invalid arguments");
     GUARD_BEGIN
     mlp::mlp_forward(params, norm, X, n, Y);
+    return NLFEM_OK;
+    GUARD_END
+}
+
+extern "C" int nlfem_mlp_forward_noalloc(const float* d_params, const float* d_norm, const float* d_X, long long n,
+                                         float* d_Y, void* stream) {
+    if (!d_params || !d_norm || (n > 0 && (!d_X || !d_Y))) return fail(NLFEM_EINVAL, "invalid arguments");
+    GUARD_BEGIN
+    mlp::mlp_forward_noalloc(d_params, d_norm, d_X, n, d_Y, (hipStream_t)stream);
     return NLFEM_OK;
     GUARD_END
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <numeric>
 #include <random>
@@ -350,6 +351,62 @@ void mlp_forward(const float* params, const float* norm, const float* X, long lo
         throw;
     }
     d.release(); (void)hipFree(dX); (void)hipFree(dY);
+}
+
+// ---------------------------------------------------------------------------- allocation-free forward
+// nlfem_mlp_forward_noalloc keeps the packed network in this file-scope device array instead of
+// hipMalloc'd buffers: forward fragments, the transposed fragments that the packing kernel also
+// writes, padded biases.  Every device holds its own copy until the process ends.  In host code
+// the array's name is only a host-side stand-in: kernels get the device address that
+// hipGetSymbolAddress returns for the current device (scratch_ptr()).
+constexpr size_t SCR_WF = 0, SCR_WB = SCR_WF + (size_t)NFRAG_FWD * 256 * sizeof(uint16_t),
+                 SCR_BIAS = SCR_WB + (size_t)NFRAG_BWD * 256 * sizeof(uint16_t),
+                 SCRATCH_BYTES = SCR_BIAS + (size_t)BIAS_PAD * sizeof(float);
+static_assert(SCR_WB % 256 == 0 && SCR_BIAS % 256 == 0 && SCRATCH_BYTES % 4 == 0, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr long long NOALLOC_MAX_SAMPLES = 2147483647LL;
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[64] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+static unsigned char* scratch_ptr() {
+    int dev = -1;
+    HIPCHECK_MLP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHECK_MLP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = p;
+    }
+    return (unsigned char*)g_scratch_dev[dev];
+}
+
+__global__ void k_zero_words(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+void mlp_forward_noalloc(const float* d_params, const float* d_norm, const float* d_X, long long n, float* d_Y,
+                         hipStream_t st) {
+    if (n < 0 || n > NOALLOC_MAX_SAMPLES)
+        throw std::invalid_argument("nlfem_mlp_forward_noalloc: n must lie in [0, 2^31 - 1]");
+    if (n == 0) return;
+    unsigned char* s = scratch_ptr();
+    uint16_t* wf = (uint16_t*)(s + SCR_WF);
+    uint16_t* wb = (uint16_t*)(s + SCR_WB);
+    float* bias = (float*)(s + SCR_BIAS);
+    // the packing kernel writes the weights and biases only: padding entries must read zero
+    // (a fill kernel, not a memset node, so that the sequence can be captured in a graph)
+    const int words = (int)(SCRATCH_BYTES / 4);
+    k_zero_words<<<(words + 255) / 256, 256, 0, st>>>((uint32_t*)s, words);
+    k_mlp_update<<<(NPARAM + 255) / 256, 256, 0, st>>>(0, nullptr, const_cast<float*>(d_params), nullptr, nullptr,
+                                                       nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, wf, wb, bias);
+    const long long tiles = (n + 15) / 16;
+    k_mlp_forward<<<(unsigned)((tiles + 3) / 4), 256, 0, st>>>(d_X, n, d_norm, (const bf16x4*)wf, bias, d_Y);
+    HIPCHECK_MLP(hipGetLastError());
 }
 
 // Gradient of the mean squared (normalised) error over all n samples, no update.
