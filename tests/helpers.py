@@ -1,4 +1,6 @@
 """Shared test helpers: reference configurations and matrix comparison."""
+import os
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -34,6 +36,60 @@ def case(name):
         return (unit_square(14, 0.12, jitter=0.25, seed=11), {"function": "constant", "horizon": 0.12, "outputdim": 1},
                 make_configuration(caps=0))
     raise KeyError(name)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gmsh_case(name, set_art_vertex=True):
+    """The reference tutorial's gmsh meshes (examples/tutorial/data/*.msh, copied as fixtures
+    under tests/golden/meshes) with the artificial vertex joining every boundary line
+    (examples/tutorial/mesh.py:23-62), and the tutorial's configurations
+    (examples/tutorial/conf.py): hole and disjoint with the constant kernel, delta = 0.1, 7/7
+    rules, CG (the tutorial's disjoint case uses the labeledValve kernel, which is not on this
+    path); circle with the microelastic field kernel, delta = 0.1, 16/7 rules, close pairs by
+    retriangulate (the tutorial's "peridynamics" case)."""
+    from nlfem_amd.gmsh import read_gmsh
+    from nlfem_amd.nlfem import meshFromArrays
+    v, e, lab = read_gmsh(os.path.join(GOLDEN, "meshes", f"{name}.msh"), set_art_vertex=set_art_vertex)
+    if name == "circle":
+        kernel = {"function": "linearPrototypeMicroelasticField", "horizon": 0.1, "outputdim": 2,
+                  "fractional_s": -0.5}
+        conf = make_configuration(close="retriangulate", outer="16point", inner="7point")
+    else:
+        kernel = {"function": "constant", "horizon": 0.1, "outputdim": 1}
+        conf = make_configuration()
+    mesh = meshFromArrays(e, lab, v, kernel["outputdim"])
+    return mesh, kernel, conf
+
+
+def fan_mesh(n=140, delta=0.3, ring=True):
+    """A disc of n triangles around one centre vertex (valence n), optionally surrounded by a
+    second ring of 2n triangles: dual-graph rows of about n elements and a node row merged
+    from n partial rows.  Inner triangles are labelled +1 (Omega), the outer ring -1."""
+    from nlfem_amd.mesh import mesh_dict
+    t = 2 * np.pi * np.arange(n) / n
+    r1 = 0.5
+    verts = [[0.0, 0.0]] + [[r1 * np.cos(a), r1 * np.sin(a)] for a in t]
+    el = [[0, 1 + i, 1 + (i + 1) % n] for i in range(n)]
+    lab = [1] * n
+    if ring:
+        r2 = 0.5 + 0.15
+        verts += [[r2 * np.cos(a + np.pi / n), r2 * np.sin(a + np.pi / n)] for a in t]
+        for i in range(n):
+            a0, a1, b0 = 1 + i, 1 + (i + 1) % n, 1 + n + i
+            el.append([a0, b0, a1])
+            el.append([a1, b0, 1 + n + (i + 1) % n])
+            lab += [-1, -1]
+    verts = np.array(verts)
+    el = np.array(el, dtype=np.int64)
+    E = verts[el]
+    area = (E[:, 1, 0] - E[:, 0, 0]) * (E[:, 2, 1] - E[:, 0, 1]) - (E[:, 2, 0] - E[:, 0, 0]) * (E[:, 1, 1] - E[:, 0, 1])
+    assert np.all(area > 0)
+    lab = np.array(lab, dtype=np.int64)
+    vl = np.where(np.isin(np.arange(len(verts)), el[lab > 0].ravel()), 1, -1).astype(np.int64)
+    mesh = mesh_dict(el, lab, verts, vl)
+    return mesh, {"function": "constant", "horizon": delta, "outputdim": 1}, make_configuration()
 
 
 def label0_mesh():
@@ -99,3 +155,30 @@ FLOOR_MAX_RATIO = 1e-10
 def check_floor_usage(name, n_floor, worst_floorless, nnz):
     assert n_floor <= FLOOR_MAX_FRACTION * nnz, f"{name}: {n_floor} of {nnz} entries rely on the absolute floor"
     assert worst_floorless <= FLOOR_MAX_RATIO, f"{name}: floorless ratio {worst_floorless:.3e}"
+
+
+def rigid_motions(mesh):
+    """t_x, t_y and u_rot = (-x1, x0) in the reference's interleaved outdim-2 numbering
+    (degree of freedom 2 node + component, src/Cassemble.cpp:736-766)."""
+    x = np.asarray(mesh["vertices"])
+    nV = x.shape[0]
+    tx, ty, ur = np.zeros(2 * nV), np.zeros(2 * nV), np.zeros(2 * nV)
+    tx[0::2] = 1.0
+    ty[1::2] = 1.0
+    ur[0::2] = -x[:, 1]
+    ur[1::2] = x[:, 0]
+    return {"t_x": tx, "t_y": ty, "u_rot": ur}
+
+
+def rigid_residuals(A, Abs, mesh):
+    """max over assembled rows of |(A u)_r| / (|A| |u|)_r for each rigid motion u, with |A| the
+    matrix of absolute addends (the scale of the rounding an entry can carry)."""
+    A = sp.csr_matrix(A)
+    Abs = sp.csr_matrix(Abs)
+    rows = np.where(mesh["nodeLabels"] > 0)[0]
+    out = {}
+    for name, u in rigid_motions(mesh).items():
+        r = (A @ u)[rows]
+        s = (Abs @ np.abs(u))[rows]
+        out[name] = float(np.max(np.abs(r) / np.maximum(s, 1e-300)))
+    return out

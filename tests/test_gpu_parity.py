@@ -14,7 +14,8 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
-from helpers import case, check_floor_usage, compare, floor_usage, label0_mesh, sorted_lists
+from helpers import (case, check_floor_usage, compare, fan_mesh, floor_usage, gmsh_case, label0_mesh, rigid_residuals,
+                     sorted_lists)
 from nlfem_amd import nlfem
 
 pytestmark = pytest.mark.gpu
@@ -79,7 +80,35 @@ def _check_case(name, mesh, kernel, conf, tol=FP64_TOL, paths=0):
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s", "C5s", "DG", "bary", "weak", "nocaps"])
 def test_parity_cases(name):
     mesh, kernel, conf = case(name)
-    _check_case(name, mesh, kernel, conf)
+    A, ref = _check_case(name, mesh, kernel, conf)
+    if kernel["outputdim"] == 2:
+        _check_rigid(name, A, mesh, kernel, conf)
+
+
+def _check_rigid(name, A, mesh, kernel, conf):
+    """The GPU matrix annihilates the rigid motions in the reference's interleaved numbering."""
+    res = rigid_residuals(A, oracle.abs_matrix(mesh, kernel, conf), mesh)
+    print(f"[parity] case={name} rigid motions: {res}", flush=True)
+    assert max(res.values()) < 1e-12, res
+
+
+@pytest.mark.parametrize("name,art", [("hole", True), ("disjoint", True), ("disjoint", False), ("circle", True)])
+def test_parity_tutorial_gmsh_meshes(name, art):
+    """The reference tutorial's own meshes (examples/tutorial/data/*.msh) with the artificial
+    vertex that joins every boundary line (128, 224 and 146 label-0 elements sharing one
+    vertex, examples/tutorial/mesh.py:23-62): neighbour lists, pattern and values against the
+    oracle.  circle runs the microelastic field kernel (outdim 2, 16/7 rules)."""
+    mesh, kernel, conf = gmsh_case(name, set_art_vertex=art)
+    A, ref = _check_case(f"{name}{'-av' if art else ''}", mesh, kernel, conf)
+    if kernel["outputdim"] == 2:
+        _check_rigid(name, A, mesh, kernel, conf)
+
+
+def test_parity_vertex_of_valence_140():
+    """A fan of 140 triangles around one vertex: dual-graph rows of ~140 elements and a node
+    row merged from 140 partial rows (more than one merge chunk)."""
+    mesh, kernel, conf = fan_mesh(140)
+    _check_case("fan140", mesh, kernel, conf)
 
 
 def test_parity_label0_traversal():

@@ -14,7 +14,7 @@ import scipy.integrate as si
 import scipy.sparse as sp
 
 import oracle
-from helpers import case, label0_mesh
+from helpers import case, fan_mesh, gmsh_case, label0_mesh
 from nlfem_amd.config import make_configuration
 from nlfem_amd.mesh import mesh_dict, unit_square
 from nlfem_amd.quadrature import P7, W7, P16, W16, tensorgauss
@@ -274,12 +274,14 @@ def _interacts(mesh, conf, kernel, a, b):
     return r != 0
 
 
-@pytest.mark.parametrize("which", ["C1", "label0", "bary"])
+@pytest.mark.parametrize("which", ["C1", "label0", "bary", "hole"])
 def test_bfs_equals_connected_component(which):
     """Neighbour list of aT = elements that interact with aT and are reachable from aT in the
     vertex-sharing dual graph through interacting (or label-0) elements."""
     if which == "label0":
         mesh, kernel, conf = label0_mesh()
+    elif which == "hole":  # the tutorial mesh with its artificial vertex (128 label-0 elements)
+        mesh, kernel, conf = gmsh_case("hole")
     elif which == "bary":
         mesh, kernel, conf = case("bary")
         mesh = unit_square(8, 0.15, jitter=0.1, seed=1)
@@ -311,20 +313,24 @@ def test_bfs_equals_connected_component(which):
 
 
 # ------------------------------------------------------------------ assembled matrices
-@pytest.mark.parametrize("name", ["C1", "C2s", "C3s", "C5s", "DG", "bary", "weak", "nocaps"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "C3s", "C5s", "DG", "bary", "weak", "nocaps", "hole", "disjoint",
+                                  "circle", "fan"])
 def test_matrix_invariants(name):
     """A 1 = 0 on every assembled row (partition of unity in all four blocks and in the
     basis differences), and the Omega x Omega block is symmetric for these kernels."""
-    mesh, kernel, conf = case(name)
+    if name == "fan":
+        mesh, kernel, conf = fan_mesh(140)
+    else:
+        mesh, kernel, conf = gmsh_case(name) if name in ("hole", "disjoint", "circle") else case(name)
     r = oracle.par_system(mesh, kernel, conf)
     A = r["A"]
     rows = np.where(mesh["nodeLabels"] > 0)[0]
     scale = abs(A).max()
-    tol = 1e-8 if name == "C5s" else 1e-13  # C5s: near-coincident outer/inner points on identical pairs
+    tol = 1e-8 if name in ("C5s", "circle") else 1e-13  # C5s: near-coincident outer/inner points on identical pairs
     assert np.abs(np.asarray(A[rows].sum(axis=1))).max() <= tol * scale
     if name not in ("DG", "bary"):  # baryCenter pairs integrate one direction only (factor 2, no Prime blocks)
         AO = A[rows][:, rows].toarray()
-        assert np.abs(AO - AO.T).max() <= (1e-8 if name == "C5s" else 1e-13) * scale
+        assert np.abs(AO - AO.T).max() <= (1e-8 if name in ("C5s", "circle") else 1e-13) * scale
 
 
 def test_thread_count_independent():

@@ -18,37 +18,16 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import case
+from helpers import case, gmsh_case, rigid_residuals
 from nlfem_amd.config import make_configuration
 from nlfem_amd.mesh import unit_square
 
 FIELD = {"function": "linearPrototypeMicroelasticField", "outputdim": 2, "fractional_s": -0.5}
 
 
-def _rigid(mesh):
-    """t_x, t_y and u_rot in the reference's interleaved numbering (2 node + component)."""
-    x = np.asarray(mesh["vertices"])
-    nV = x.shape[0]
-    tx, ty, ur = np.zeros(2 * nV), np.zeros(2 * nV), np.zeros(2 * nV)
-    tx[0::2] = 1.0
-    ty[1::2] = 1.0
-    ur[0::2] = -x[:, 1]
-    ur[1::2] = x[:, 0]
-    return {"t_x": tx, "t_y": ty, "u_rot": ur}
-
-
 def _residuals(mesh, kernel, conf):
-    """max over assembled rows of |(A u)_r| / (|A| |u|)_r for each rigid motion u, where |A| is
-    the oracle's matrix of absolute addends (the scale of the rounding an entry can carry)."""
-    A = oracle.par_system(mesh, kernel, conf)["A"].tocsr()
-    Abs = oracle.abs_matrix(mesh, kernel, conf).tocsr()
-    rows = np.where(mesh["nodeLabels"] > 0)[0]
-    out = {}
-    for name, u in _rigid(mesh).items():
-        r = (A @ u)[rows]
-        s = (Abs @ np.abs(u))[rows]
-        out[name] = float(np.max(np.abs(r) / np.maximum(s, 1e-300)))
-    return out
+    A = oracle.par_system(mesh, kernel, conf)["A"]
+    return rigid_residuals(A, oracle.abs_matrix(mesh, kernel, conf), mesh)
 
 
 def _cases():
@@ -63,7 +42,8 @@ def _cases():
     kn["horizon"] = 0.2
     nocaps = (unit_square(8, 0.2, jitter=0.15, seed=6, outdim=2), kn,
               make_configuration(close="retriangulate", outer="16point", inner="7point", caps=0))
-    return {"C5s": c5s, "cut-dominated": cut, "nocaps": nocaps}
+    # the reference tutorial's circle mesh with its artificial vertex (146 label-0 elements)
+    return {"C5s": c5s, "cut-dominated": cut, "nocaps": nocaps, "circle": gmsh_case("circle")}
 
 
 CASES = _cases()
