@@ -161,8 +161,12 @@ static nlfem_plan* make_plan(int K, const long* tri, const long* lab, const doub
         delete h;
         throw;
     }
-    // geometric estimates for the BFS / pattern capacities
+    // geometric estimates for the BFS / pattern capacities, over the elements that are
+    // integrated: label-0 elements are only traversed (src/Cassemble.cpp:630-633,773-775), so
+    // they may be degenerate (the reference tutorial's artificial elements join boundary lines
+    // to one far vertex, some of them with zero area) and do not size any table
     double hmax = 0, amin = 1e300, asum = 0;
+    long nlab = 0;
     for (int e = 0; e < nE; e++) {
         const long* t = tri + 3 * (size_t)e;
         double E[6];
@@ -170,6 +174,8 @@ static nlfem_plan* make_plan(int K, const long* tri, const long* lab, const doub
             if (t[j] < 0 || t[j] >= nV) { delete h; throw std::invalid_argument("element vertex index out of range"); }
             E[2 * j] = verts[2 * t[j]]; E[2 * j + 1] = verts[2 * t[j] + 1];
         }
+        if (lab[e] == 0) continue;
+        nlab++;
         for (int j = 0; j < 3; j++) {
             int k = (j + 1) % 3;
             double d = std::hypot(E[2 * k] - E[2 * j], E[2 * k + 1] - E[2 * j + 1]);
@@ -180,9 +186,10 @@ static nlfem_plan* make_plan(int K, const long* tri, const long* lab, const doub
         amin = std::min(amin, a);
         asum += a;
     }
+    if (nlab == 0) { delete h; throw std::invalid_argument("no element with a nonzero label"); }
     h->est.hmax = hmax;
     h->est.amin = amin;
-    h->est.amean = nE > 0 ? asum / nE : 1.0;
+    h->est.amean = asum / nlab;
     try {
         plan_setup(h->P, tri, lab, verts, vlab);
     } catch (...) {
