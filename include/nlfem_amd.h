@@ -30,8 +30,11 @@ const char* nlfem_last_error(void);
  * src/Cassemble.cpp:850-851); "forcing" writes the load vector as ARMA_MAT_BIN_FN008 to
  * path_fd (src/Cassemble.cpp:927).  std::string arguments become NUL-terminated strings.
  * K_Omega, nE_Omega (checked), nV_Omega, is_NeumannBoundary, str_model_f (system),
- * maxDiameter and verbose keep the reference's (lack of) effect on values.  Only dim == 2 and
- * nZeta == 0 are supported (NLFEM_EINVAL otherwise). */
+ * maxDiameter and verbose keep the reference's (lack of) effect on values.  Only dim == 2 is
+ * supported (NLFEM_EINVAL otherwise).  nZeta > 0 applies the domain-decomposition weights
+ * 1 / |Z_aT n Z_bT| of the CSR indicator (ptrZetaIndicator_indices, _indptr; nE rows, nZeta
+ * columns) to every pair, and 1 / |Z_aT| to every load contribution (src/mathhelpers.cpp:19-38,
+ * src/Cassemble.cpp:648-656,906-914). */
 int nlfem_par_assemble(const char* compute, const char* path_spAd, const char* path_fd, int K_Omega, int K,
                        const long* ptrTriangles, const long* ptrLabelTriangles, const double* ptrVerts,
                        const long* ptrLabelVerts, int nE, int nE_Omega, int nV, int nV_Omega, const double* Px,
@@ -140,10 +143,20 @@ int nlfem_plan_surrogate_samples(nlfem_plan* plan, long long maxn, unsigned stri
                                  double smax, float* feat, float* targ, long long* count);
 void nlfem_plan_destroy(nlfem_plan* plan);
 
-/* Load vector on the device (par_forcing, src/Cassemble.cpp:857-928): fd[K]. */
+/* Load vector on the device (par_forcing, src/Cassemble.cpp:857-928): fd[K].  nZeta > 0 weighs
+ * element aT's contribution by 1 / |Z_aT| (src/Cassemble.cpp:906-914); nZeta = 0 ignores the
+ * indicator arrays (may be NULL). */
 int nlfem_load_vector(int K, const long* ptrTriangles, const long* ptrLabelTriangles, const double* ptrVerts,
                       const long* ptrLabelVerts, int nE, int nV, const double* Px, int nPx, const double* dx,
-                      int is_DiscontinuousGalerkin, const char* str_model_f, int outdim, double* fd);
+                      int is_DiscontinuousGalerkin, const char* str_model_f, int outdim, double* fd,
+                      const long* ptrZetaIndicator_indices, const long* ptrZetaIndicator_indptr, long nZeta);
+/* Domain-decomposition weights of a plan (the nZeta > 0 case of par_assemble,
+ * src/mathhelpers.cpp:19-38 applied at src/Cassemble.cpp:648-656,741-766): indicator rows
+ * indices[indptr[e] .. indptr[e+1]) of the nE elements, columns < nZeta; every pair's four
+ * blocks are scaled by 1 / |Z_aT n Z_bT| (an empty intersection gives inf, as in the reference).
+ * nZeta = 0 removes them.  Weighted plans integrate every pair with the per-pair kernels
+ * (slower than the default path) and do not use the surrogate. */
+int nlfem_plan_set_zeta(nlfem_plan* plan, const long* indices, const long* indptr, long nZeta);
 
 /* ------------------------------------------------------------------------------------
  * Cut-pair surrogate network (no counterpart in the reference): a 12 -> 64 -> 64 -> 64 -> 9

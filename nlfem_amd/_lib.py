@@ -43,7 +43,8 @@ SIGNATURES = {
     "nlfem_plan_stats": (C.c_int, [C.c_void_p, _dp, C.c_int]),
     "nlfem_plan_destroy": (None, [C.c_void_p]),
     "nlfem_load_vector": (C.c_int, [C.c_int, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _cp,
-                                    C.c_int, _dp]),
+                                    C.c_int, _dp, _lp, _lp, C.c_long]),
+    "nlfem_plan_set_zeta": (C.c_int, [C.c_void_p, _lp, _lp, C.c_long]),
     "nlfem_plan_set_estimates": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "nlfem_plan_force_paths": (C.c_int, [C.c_void_p, C.c_int]),
     "nlfem_plan_set_cut_items": (C.c_int, [C.c_void_p, C.c_int]),

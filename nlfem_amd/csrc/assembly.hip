@@ -1411,7 +1411,26 @@ struct NumArgs {
     // k_lp_gather; null: added to the matrix directly
     const long long* coff;
     double* stage;
+    // domain-decomposition weights (null: 1): subdomain rows of the elements, sorted
+    const long long* zptr;
+    const int* zidx;
 };
+
+// Weight of pair (a, b) under a domain decomposition: 1 / |Z_a n Z_b|, the number of subdomains
+// the two elements share (evaluateZeta, src/mathhelpers.cpp:19-38, applied to all four blocks at
+// src/Cassemble.cpp:648-656,741-766); an empty intersection gives 1/0 = inf, as there.
+NLFEM_DEV double zeta_weight(const long long* zp, const int* zi, int a, int b) {
+    long long i = zp[a], j = zp[b];
+    const long long ie = zp[a + 1], je = zp[b + 1];
+    int n = 0;
+    while (i < ie && j < je) {
+        const int x = zi[i], y = zi[j];
+        if (x < y) i++;
+        else if (y < x) j++;
+        else { n++; i++; j++; }
+    }
+    return 1.0 / (double)n;
+}
 
 // 1 / sqrt(r2) for r2 > 0: the hardware estimate (v_rsq_f64, relative error below 2^-22)
 // refined by two Newton-Raphson steps y <- y + y (1 - r2 y^2) / 2, which leave an error of a
@@ -1794,7 +1813,9 @@ NLFEM_DEV void cut_flush_local(const NumArgs& a, int aT, const double* tL, const
 // One block of 256 threads per outer element; threads stride over the element's
 // neighbour-list segment.  SEG 0: front segment (interior pairs; close pairs are left to the
 // tensor-Gauss kernel).  SEG 1: back segment (cut pairs, or baryCenter pairs).
-template <int KER, int OD, int SEG, typename T>
+// ZW: every pair weighted by its domain-decomposition weight (zeta_weight): the pair's termLocal
+// goes through a per-pair buffer so that it can be scaled before it joins the thread's sum.
+template <int KER, int OD, int SEG, typename T, bool ZW = false>
 __global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO;
     extern __shared__ int lds[];
@@ -1843,14 +1864,30 @@ __global__ __launch_bounds__(256) void k_pairs(NumArgs a) {
         const ElemRec B = a.elem[b];
         PairBlocks<OD> pb;
         bool prime = true;
+        double tLz[ZW ? NB : 1];
+        double* tLp = tL;
+        if constexpr (ZW) {
+#pragma unroll
+            for (int i = 0; i < NB; i++) tLz[i] = 0;
+            tLp = tLz;
+        }
         if constexpr (SEG == 0) {
-            pair_interior<KER, OD, T>(a, A, B, tL, pb);
+            pair_interior<KER, OD, T>(a, A, B, tLp, pb);
         } else {
             if (e >> 31) {
-                pair_bary<KER, OD, T>(a, A, B, tL, pb);
+                pair_bary<KER, OD, T>(a, A, B, tLp, pb);
                 prime = false;
             } else {
-                pair_cut<KER, OD, T>(a, A, B, tL, pb, ntri);
+                pair_cut<KER, OD, T>(a, A, B, tLp, pb, ntri);
+            }
+        }
+        if constexpr (ZW) {
+            const double w = zeta_weight(a.zptr, a.zidx, aT, b);
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                tL[i] += w * tLz[i];
+                pb.tN[i] *= w;
+                pb.tLP[i] *= w;
             }
         }
         scatter_pair<OD>(a, rA, labA, b, prime, pb, hk, hv, useHash);
@@ -3311,6 +3348,8 @@ struct CloseArgs {
     double* val;
     KernelParams kp;
     unsigned long long* counters;
+    const long long* zptr;  // domain-decomposition weights (null: 1)
+    const int* zidx;
 };
 
 // Tensor-Gauss quadrature of a first-layer pair with a singular kernel: integrate_fractional
@@ -3411,7 +3450,8 @@ __global__ __launch_bounds__(64) void k_close(CloseArgs a) {
     }
     for (int i = lane; i < 4 * NB; i += 64) (&blk[0][0])[i] = 0;
     __syncthreads();
-    const double ab = absA * absB;
+    // every block carries |aT| |bT|, and with a domain decomposition the pair's weight
+    const double ab = absA * absB * (a.zptr ? zeta_weight(a.zptr, a.zidx, aT, bT) : 1.0);
     if (lane == 0) {
         double* tL = blk[0];
         double* tN = blk[1];
@@ -3710,6 +3750,8 @@ static NumArgs base_numargs(const Plan& P) {
     }
     na.sur = SurArgs{};
     na.sur.stride = 1;
+    na.zptr = P.d_zptr;
+    na.zidx = P.d_zidx;
     return na;
 }
 
@@ -3718,7 +3760,7 @@ static NumArgs base_numargs(const Plan& P) {
 static bool surrogate_applies(const Plan& P) {
     const PlanConfig& c = P.cfg;
     return P.sur.nnet > 0 && c.kernel == KER_CONSTANT && c.outdim == 1 && c.method_remote == M_RETRIANGULATE &&
-           !c.singular && P.stats.bary == 0 && c.q.nPx <= SUR_ITEMS / 4;
+           !c.singular && P.stats.bary == 0 && c.q.nPx <= SUR_ITEMS / 4 && !P.d_zptr;
 }
 
 static void set_surrogate_args(const Plan& P, SurArgs& s) {
@@ -4041,9 +4083,30 @@ void plan_assemble(Plan& P, const Estimates& est) {
     na.HN = (HN * 16 <= 64 * 1024) ? (int)HN : 0;
     const size_t nlds = na.HN > 0 ? (size_t)na.HN * 16 : 0;
 
+    // With domain-decomposition weights every pair is integrated and scattered by the per-pair
+    // kernel, its four blocks scaled by the pair's weight (FP64 atomics into val).
+#define LAUNCH_PAIRS_Z(KER, OD, SEG)                                                                        \
+    do {                                                                                                    \
+        if (c.precision == 1) k_pairs<KER, OD, SEG, float, true><<<c.nE, 256, nlds, st>>>(na);              \
+        else k_pairs<KER, OD, SEG, double, true><<<c.nE, 256, nlds, st>>>(na);                              \
+    } while (0)
+#define DISPATCH_PAIRS_Z(SEG)                                                      \
+    do {                                                                           \
+        switch (c.kernel) {                                                        \
+            case KER_CONSTANT: LAUNCH_PAIRS_Z(KER_CONSTANT, 1, SEG); break;        \
+            case KER_PARABOLA: LAUNCH_PAIRS_Z(KER_PARABOLA, 1, SEG); break;        \
+            case KER_FRACTIONAL: LAUNCH_PAIRS_Z(KER_FRACTIONAL, 1, SEG); break;    \
+            case KER_MICRO: LAUNCH_PAIRS_Z(KER_MICRO, 1, SEG); break;              \
+            default: LAUNCH_PAIRS_Z(KER_MICRO_FIELD, 2, SEG); break;               \
+        }                                                                          \
+    } while (0)
+
     // interior pairs: owner computes into partial rows, then merge (plain stores into val)
     ev_begin(P, PH_INTERIOR);
-    {
+    if (P.d_zptr) {
+        DISPATCH_PAIRS_Z(0);
+        HIPCHECK(hipGetLastError());
+    } else {
         int*& d_plen = P.d_wplen;
         dreserve(P, d_plen, 3 * (size_t)c.nE);
         dreserve(P, P.d_poff, 3 * (size_t)c.nE + 1);
@@ -4114,6 +4177,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ca.is_DG = c.is_DG; ca.nPg = c.nPg; ca.od = c.outdim; ca.elem = P.d_elem; ca.tri = P.d_tri;
         ca.vlabel = P.d_vlabel; ca.list = P.d_closeList; ca.tg = P.d_tg; ca.dg = P.d_dg; ca.rowptr = P.d_rowptr;
         ca.col = P.d_col; ca.val = P.d_val; ca.kp = na.kp; ca.counters = P.d_counters;
+        ca.zptr = P.d_zptr; ca.zidx = P.d_zidx;
         const bool weak = c.method_close == M_WEAKSINGULAR;
 #define LAUNCH_CLOSE(KER, OD)                                                                     \
     do {                                                                                          \
@@ -4152,7 +4216,9 @@ void plan_assemble(Plan& P, const Estimates& est) {
     bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
-    if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
+    if (P.d_zptr) {
+        DISPATCH_PAIRS_Z(1);
+    } else if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
         DISPATCH_PAIRS(1);
     } else {
         NumArgs nc = na;
@@ -4266,6 +4332,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
     ev_end(P, PH_GATHER);
 #undef DISPATCH_PAIRS
 #undef LAUNCH_PAIRS
+#undef DISPATCH_PAIRS_Z
+#undef LAUNCH_PAIRS_Z
 
     // ---------------------------------------------------------------- compaction
     ev_begin(P, PH_COMPACT);
@@ -4379,6 +4447,37 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
     }
 }
 
+// Domain-decomposition indicator (mesh["ZetaIndicator"], cython/nlfem.pyx:205-214): CSR rows of
+// the subdomains of each element; rows are sorted here (the reference's set_intersection
+// expects sorted rows).  nZeta == 0 removes it.
+void plan_set_zeta(Plan& P, const long* indices, const long* indptr, long nZeta) {
+    const int nE = P.cfg.nE;
+    if (P.d_zptr) { (void)hipFree(P.d_zptr); P.d_zptr = nullptr; }
+    if (P.d_zidx) { (void)hipFree(P.d_zidx); P.d_zidx = nullptr; }
+    P.nZeta = 0;
+    if (nZeta <= 0) return;
+    if (!indices || !indptr) throw std::invalid_argument("ZetaIndicator arrays missing");
+    if (indptr[0] != 0) throw std::invalid_argument("ZetaIndicator indptr must start at 0");
+    std::vector<long long> zp((size_t)nE + 1);
+    for (int e = 0; e <= nE; e++) {
+        zp[e] = indptr[e];
+        if (e > 0 && zp[e] < zp[e - 1]) throw std::invalid_argument("ZetaIndicator indptr must be non-decreasing");
+    }
+    if (zp[nE] >= (1ll << 31)) throw std::invalid_argument("ZetaIndicator has too many entries");
+    std::vector<int> zi((size_t)std::max<long long>(zp[nE], 1));
+    for (long long i = 0; i < zp[nE]; i++) {
+        if (indices[i] < 0 || indices[i] >= nZeta) throw std::invalid_argument("ZetaIndicator column out of range");
+        zi[i] = (int)indices[i];
+    }
+    for (int e = 0; e < nE; e++) std::sort(zi.begin() + zp[e], zi.begin() + zp[e + 1]);
+    dalloc(P.d_zptr, (size_t)nE + 1);
+    dalloc(P.d_zidx, zi.size());
+    HIPCHECK(hipMemcpyAsync(P.d_zptr, zp.data(), sizeof(long long) * zp.size(), hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipMemcpyAsync(P.d_zidx, zi.data(), sizeof(int) * zi.size(), hipMemcpyHostToDevice, P.stream));
+    HIPCHECK(hipStreamSynchronize(P.stream));
+    P.nZeta = nZeta;
+}
+
 void plan_free(Plan& P) {
     void* sp[] = {P.sur.d_wf, P.sur.d_bias, P.sur.d_norm, P.d_rlist, P.d_roff, P.d_rcnt};
     for (void* p : sp)
@@ -4386,7 +4485,8 @@ void plan_free(Plan& P) {
     void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
-                    P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF};
+                    P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF, P.d_zptr,
+                    P.d_zidx};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     void* wp[] = {P.d_wcnt, P.d_wrowcnt, P.d_wptmp, P.d_wrowOf, P.d_wplen, P.d_wocnt};

@@ -37,8 +37,9 @@ struct AuxQuad {
 
 // par_forcing (src/Cassemble.cpp:857-928) with compute_f (:149-164): one thread per element,
 // FP64 atomics into the load vector.
+// weight: per-element domain-decomposition weight 1 / |Z_aT| (src/Cassemble.cpp:906-914), or null.
 __global__ void k_forcing(int nE, const long* tri, const long* lab, const double* verts, const long* vlab,
-                          int is_DG, int od, int f, AuxQuad q, double* fd) {
+                          int is_DG, int od, int f, AuxQuad q, const double* weight, double* fd) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nE || lab[e] <= 0) return;
     double E[6];
@@ -60,8 +61,9 @@ __global__ void k_forcing(int nE, const long* tri, const long* lab, const double
             const double psi[3] = {1 - p0 - p1, p0, p1};
             termf[a] += psi[a / od] * fv[a % od] * det * q.w[i];
         }
+    const double w = weight ? weight[e] : 1.0;
     for (int a = 0; a < 3 * od; a++)
-        if (is_DG || vlab[node[a / od]] > 0) atomicAdd(&fd[od * node[a / od] + a % od], termf[a]);
+        if (is_DG || vlab[node[a / od]] > 0) atomicAdd(&fd[od * node[a / od] + a % od], termf[a] * w);
 }
 
 // par_evaluateMass (src/Cassemble.cpp:166-246): v = M u over the first J elements.
@@ -151,22 +153,25 @@ static AuxQuad aux_quad(const double* P, const double* w, int nP) {
 }
 
 void run_forcing(int K, const long* tri, const long* lab, const double* verts, const long* vlab, int nE, int nV,
-                 const double* Px, int nPx, const double* dx, int is_DG, int od, int f, double* fd_host) {
+                 const double* Px, int nPx, const double* dx, int is_DG, int od, int f, const double* weight,
+                 double* fd_host) {
     hipStream_t st;
     HIPCHECK_AUX(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     long* d_tri = up(tri, 3 * (size_t)nE, st);
     long* d_lab = up(lab, (size_t)nE, st);
     double* d_verts = up(verts, 2 * (size_t)nV, st);
     long* d_vlab = up(vlab, (size_t)nV, st);
+    double* d_w = weight ? up(weight, (size_t)nE, st) : nullptr;
     double* d_fd;
     HIPCHECK_AUX(hipMalloc(&d_fd, sizeof(double) * (size_t)K));
     HIPCHECK_AUX(hipMemsetAsync(d_fd, 0, sizeof(double) * (size_t)K, st));
     k_forcing<<<(nE + 255) / 256, 256, 0, st>>>(nE, d_tri, d_lab, d_verts, d_vlab, is_DG, od, f,
-                                                aux_quad(Px, dx, nPx), d_fd);
+                                                aux_quad(Px, dx, nPx), d_w, d_fd);
     HIPCHECK_AUX(hipGetLastError());
     HIPCHECK_AUX(hipMemcpyAsync(fd_host, d_fd, sizeof(double) * (size_t)K, hipMemcpyDeviceToHost, st));
     HIPCHECK_AUX(hipStreamSynchronize(st));
     (void)hipFree(d_tri); (void)hipFree(d_lab); (void)hipFree(d_verts); (void)hipFree(d_vlab); (void)hipFree(d_fd);
+    if (d_w) (void)hipFree(d_w);
     (void)hipStreamDestroy(st);
 }
 

@@ -21,7 +21,9 @@ void plan_set_surrogate(Plan& P, int nnet, const float* params, const float* nor
 long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsigned salt, double smin, double smax,
                                  float* feat, float* targ);
 void run_forcing(int K, const long* tri, const long* lab, const double* verts, const long* vlab, int nE, int nV,
-                 const double* Px, int nPx, const double* dx, int is_DG, int od, int f, double* fd_host);
+                 const double* Px, int nPx, const double* dx, int is_DG, int od, int f, const double* weight,
+                 double* fd_host);
+void plan_set_zeta(Plan& P, const long* indices, const long* indptr, long nZeta);
 void run_mass(int K, const long* tri, const long* lab, const double* verts, const long* vlab, int J, int nE, int nV,
               const double* P, int nP, const double* dx, int is_DG, int od, const double* ud, double* vd_host);
 void run_retriangulate(int n, const double* x, const double* TE, double sqdelta, int caps, double* lists, int* ntri);
@@ -413,13 +415,29 @@ static int write_arma_mat(const std::string& path, long long K, const std::vecto
 
 extern "C" int nlfem_load_vector(int K, const long* tri, const long* lab, const double* verts, const long* vlab,
                                  int nE, int nV, const double* Px, int nPx, const double* dx, int is_DG,
-                                 const char* fname, int outdim, double* fd) {
+                                 const char* fname, int outdim, double* fd, const long* zi, const long* zp,
+                                 long nZeta) {
     GUARD_BEGIN
     int f = force_id(fname ? fname : "");
     if (f < 0) return fail(NLFEM_EINVAL, std::string("unknown forcing '") + (fname ? fname : "") + "'");
     if ((f == 2) != (outdim == 2)) return fail(NLFEM_EINVAL, "forcing output dimension does not match outdim");
     if (is_DG ? (outdim * nE * 3 != K) : (outdim * nV != K)) return fail(NLFEM_EINVAL, "K mismatch");
-    run_forcing(K, tri, lab, verts, vlab, nE, nV, Px, nPx, dx, is_DG, outdim, f, fd);
+    std::vector<double> w;
+    if (nZeta > 0) {
+        // evaluateZeta(aT, aT) is the length of aT's indicator row (src/Cassemble.cpp:906-914)
+        if (!zi || !zp) return fail(NLFEM_EINVAL, "nZeta > 0 needs the ZetaIndicator arrays");
+        w.resize((size_t)nE);
+        for (int e = 0; e < nE; e++) w[e] = 1.0 / (double)(zp[e + 1] - zp[e]);
+    }
+    run_forcing(K, tri, lab, verts, vlab, nE, nV, Px, nPx, dx, is_DG, outdim, f, w.empty() ? nullptr : w.data(), fd);
+    return NLFEM_OK;
+    GUARD_END
+}
+
+extern "C" int nlfem_plan_set_zeta(nlfem_plan* plan, const long* indices, const long* indptr, long nZeta) {
+    if (!plan) return fail(NLFEM_EINVAL, "null plan");
+    GUARD_BEGIN
+    plan_set_zeta(plan->P, indices, indptr, nZeta);
     return NLFEM_OK;
     GUARD_END
 }
@@ -434,10 +452,10 @@ extern "C" int nlfem_par_assemble(const char* compute, const char* path_spAd, co
                                   int is_PlacePointOnCap, int dim, int outdim, const long* zi, const long* zp,
                                   long nZeta, const double* Pg, int degree, const double* dg, double maxDiameter,
                                   double fractional_s, int is_fullConnectedComponentSearch, int verbose) {
-    (void)K_Omega; (void)nV_Omega; (void)is_NeumannBoundary; (void)zi; (void)zp; (void)maxDiameter;
+    (void)K_Omega; (void)nV_Omega; (void)is_NeumannBoundary; (void)maxDiameter;
     GUARD_BEGIN
     if (dim != 2) return fail(NLFEM_EINVAL, "only dim == 2 is supported");
-    if (nZeta != 0) return fail(NLFEM_EINVAL, "domain-decomposition weights (nZeta > 0) are not supported");
+    if (nZeta > 0 && (!zi || !zp)) return fail(NLFEM_EINVAL, "nZeta > 0 needs the ZetaIndicator arrays");
     std::string comp = compute ? compute : "";
     if (comp == "system") {
         long long cnt = 0;
@@ -447,7 +465,8 @@ extern "C" int nlfem_par_assemble(const char* compute, const char* path_spAd, co
                                   Py, nPy, dy, sqdelta, is_DG, str_model_kernel, str_integration_method_remote,
                                   str_integration_method_close, is_PlacePointOnCap, outdim, Pg, degree, dg,
                                   fractional_s, is_fullConnectedComponentSearch, 0);
-        int rc = nlfem_plan_assemble(h);
+        int rc = nZeta > 0 ? nlfem_plan_set_zeta(h, zi, zp, nZeta) : NLFEM_OK;
+        if (rc == NLFEM_OK) rc = nlfem_plan_assemble(h);
         if (rc == NLFEM_OK) {
             long long nnz = h->P.nnz;
             std::vector<long long> rp((size_t)K + 1);
@@ -465,7 +484,7 @@ extern "C" int nlfem_par_assemble(const char* compute, const char* path_spAd, co
     if (comp == "forcing") {
         std::vector<double> fd((size_t)K);
         int rc = nlfem_load_vector(K, ptrTriangles, ptrLabelTriangles, ptrVerts, ptrLabelVerts, nE, nV, Px, nPx, dx,
-                                   is_DG, str_model_f, outdim, fd.data());
+                                   is_DG, str_model_f, outdim, fd.data(), zi, zp, nZeta);
         if (rc != NLFEM_OK) return rc;
         return write_arma_mat(path_fd ? path_fd : "", K, fd);
     }

@@ -124,6 +124,10 @@ struct Plan {
     int* d_ocol = nullptr;
     double* d_oval = nullptr;
     long long nnz = 0;
+    // domain-decomposition indicator (nZeta > 0): CSR rows of subdomains per element, sorted
+    long long* d_zptr = nullptr;  // [nE+1], null: every pair has weight 1
+    int* d_zidx = nullptr;
+    long nZeta = 0;
     // scratch
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;

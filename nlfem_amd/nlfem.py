@@ -69,6 +69,22 @@ def _strs(*s):
     return [x.encode("utf-8") for x in s]
 
 
+def _zeta(mesh):
+    """Domain-decomposition indicator as the reference's wrapper reads it
+    (cython/nlfem.pyx:205-214): mesh["nZeta"] (0 or absent: no weights) and the CSR rows of
+    mesh["ZetaIndicator"] (elements x subdomains)."""
+    n = int(mesh.get("nZeta", 0)) if isinstance(mesh, dict) else 0
+    if n <= 0:
+        z = np.zeros(1, dtype=np.int64)
+        return z, z, 0
+    Z = mesh["ZetaIndicator"]
+    zi = np.ascontiguousarray(np.asarray(Z.indices), dtype=np.int64)
+    zp = np.ascontiguousarray(np.asarray(Z.indptr), dtype=np.int64)
+    if zp.size != mesh["elements"].shape[0] + 1:
+        raise ValueError("ZetaIndicator must have one row per element")
+    return (zi if zi.size else np.zeros(1, dtype=np.int64)), zp, n
+
+
 class Assembler:
     """Device-resident assembly plan (nlfem_plan_* in include/nlfem_amd.h).
 
@@ -93,6 +109,9 @@ class Assembler:
                                   a.is_fullConnectedComponentSearch,
                                   _PRECISION[configuration.get("precision", "fp64")]))
         self._h = h
+        zi, zp, nz = _zeta(mesh)
+        if nz > 0:
+            check(L.nlfem_plan_set_zeta(h, lp(zi), lp(zp), nz))
         sur = configuration.get("surrogate")
         if sur:
             self.set_surrogate(**(sur if isinstance(sur, dict) else {}))
@@ -218,13 +237,13 @@ def stiffnessMatrix(mesh, kernel, configuration):
     path = configuration["savePath"]
     Pg = a.Pg if a.Pg.size else np.zeros(1)
     dg = a.dg if a.dg.size else np.zeros(1)
-    zero = np.zeros(1, dtype=np.int64)
+    zi, zp, nz = _zeta(mesh)
     start = time.time()
     check(lib().nlfem_par_assemble(
         *_strs("system", path, ""), a.K_Omega, a.K, lp(a.elements), lp(a.elementLabels), dp(a.vertices),
         lp(a.vertexLabels), a.nE, a.nE_Omega, a.nV, a.nV_Omega, dp(a.Px), a.Px.size // 2, dp(a.dx), dp(a.Py),
         a.Py.size // 2, dp(a.dy), a.sqdelta, a.is_DG, 0, *_strs(a.kernel, ""),
-        *_strs(a.method_remote, a.method_close), a.is_PlacePointOnCap, a.dim, a.outdim, lp(zero), lp(zero), 0,
+        *_strs(a.method_remote, a.method_close), a.is_PlacePointOnCap, a.dim, a.outdim, lp(zi), lp(zp), nz,
         dp(Pg), a.degree, dp(dg), a.maxDiameter, a.fractional_s, a.is_fullConnectedComponentSearch, verbose))
     if verbose:
         print("Assembly Time\t", "{:1.2e}".format(time.time() - start), " Sec")
@@ -236,9 +255,10 @@ def loadVector(mesh, load, configuration):
     kern = {"function": "constant", "horizon": 1.0, "outputdim": mesh.get("outdim", 1)}
     a = parse(mesh, kern, configuration)
     fd = np.zeros(a.K)
+    zi, zp, nz = _zeta(mesh)
     check(lib().nlfem_load_vector(a.K, lp(a.elements), lp(a.elementLabels), dp(a.vertices), lp(a.vertexLabels),
                                   a.nE, a.nV, dp(a.Px), a.Px.size // 2, dp(a.dx), a.is_DG,
-                                  load["function"].encode("utf-8"), a.outdim, dp(fd)))
+                                  load["function"].encode("utf-8"), a.outdim, dp(fd), lp(zi), lp(zp), nz))
     return fd
 
 

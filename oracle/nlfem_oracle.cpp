@@ -672,6 +672,18 @@ ForceFn lookupForce(const std::string& s) {
 }  // namespace
 
 // ================================================================ C ABI (ctypes)
+// src/mathhelpers.cpp:19-38 evaluateZeta: size of the intersection of the indicator rows of
+// aTdx and bTdx (both sorted ascending, as std::set_intersection requires).
+static double evaluateZeta(const long* indices, const long* indptr, long aTdx, long bTdx) {
+    long i = indptr[aTdx], ie = indptr[aTdx + 1], j = indptr[bTdx], je = indptr[bTdx + 1], n = 0;
+    while (i < ie && j < je) {
+        if (indices[i] < indices[j]) i++;
+        else if (indices[j] < indices[i]) j++;
+        else { n++; i++; j++; }
+    }
+    return (double)n;
+}
+
 extern "C" {
 
 struct OracleResult {
@@ -701,7 +713,8 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                 const char* kernel, const char* method_remote, const char* method_close,
                                 int caps, int outdim, const double* Pg, int degree, const double* dg,
                                 double fractional_s, int fullCC, int nthreads, int keep_neighbours,
-                                int outer_begin, int outer_end) {
+                                int outer_begin, int outer_end, const long* zeta_indices, const long* zeta_indptr,
+                                long nZeta) {
     Mesh m;
     m.nE = nE; m.nV = nV; m.outdim = outdim; m.K = K; m.is_DG = is_DG;
     m.tri = tri; m.triLabel = triLabel; m.verts = verts; m.vertLabel = vertLabel;
@@ -789,6 +802,9 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                 queue.push(bTdx);
                                 // src/Cassemble.cpp:721-770 scatter; degree of freedom (node, component) is
                                 // od * node + component (interleaved, src/Cassemble.cpp:736-766)
+                                // domain decomposition, src/Cassemble.cpp:648-656: weight 1 / zeta
+                                const double weight =
+                                    nZeta ? 1. / evaluateZeta(zeta_indices, zeta_indptr, aTdx, bTdx) : 1.;
                                 auto dof = [&](long node, int comp) -> unsigned long {
                                     if (g_plant & PLANT_BLOCKED_LAYOUT) return (unsigned long)comp * (K / od) + node;
                                     return (unsigned long)(od * node + comp);
@@ -798,15 +814,15 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                         unsigned long Adx;
                                         if (is_DG || vertLabel[aAdx[a / od]] > 0) {
                                             Adx = dof(aAdx[a / od], a % od) * K + dof(aAdx[b / od], b % od);
-                                            Ad[Adx] += g_abs ? std::fabs(tL[dV * od * a + b]) : tL[dV * od * a + b];
+                                            Ad[Adx] += (g_abs ? std::fabs(tL[dV * od * a + b]) : tL[dV * od * a + b]) * weight;
                                             Adx = dof(aAdx[a / od], a % od) * K + dof(bAdx[b / od], b % od);
-                                            Ad[Adx] += g_abs ? std::fabs(tN[dV * od * a + b]) : -tN[dV * od * a + b];
+                                            Ad[Adx] += (g_abs ? std::fabs(tN[dV * od * a + b]) : -tN[dV * od * a + b]) * weight;
                                         }
                                         if (is_DG || vertLabel[bAdx[b / od]] > 0) {
                                             Adx = dof(bAdx[b / od], b % od) * K + dof(bAdx[a / od], a % od);
-                                            Ad[Adx] += g_abs ? std::fabs(tLP[dV * od * a + b]) : tLP[dV * od * a + b];
+                                            Ad[Adx] += (g_abs ? std::fabs(tLP[dV * od * a + b]) : tLP[dV * od * a + b]) * weight;
                                             Adx = dof(bAdx[b / od], b % od) * K + dof(aAdx[a / od], a % od);
-                                            Ad[Adx] += g_abs ? std::fabs(tNP[dV * od * a + b]) : -tNP[dV * od * a + b];
+                                            Ad[Adx] += (g_abs ? std::fabs(tNP[dV * od * a + b]) : -tNP[dV * od * a + b]) * weight;
                                         }
                                     }
                                 }
@@ -947,7 +963,7 @@ int oracle_quadrule(const double* Pg, int degree, double s, int cs, double* alph
 // src/Cassemble.cpp:857-928 par_forcing (+ compute_f :149-164)
 int oracle_par_forcing(int K, const long* tri, const long* triLabel, const double* verts, const long* vertLabel,
                        int nE, const double* Px, int nPx, const double* dx, int is_DG, const char* force, int outdim,
-                       double* fd) {
+                       double* fd, const long* zeta_indices, const long* zeta_indptr, long nZeta) {
     ForceFn f = lookupForce(force);
     if (!f) return -1;
     std::vector<double> psix(3 * nPx);
@@ -970,10 +986,12 @@ int oracle_par_forcing(int K, const long* tri, const long* triLabel, const doubl
                 f(x, fv);
                 termf[a] += psix[3 * i + a / outdim] * fv[a % outdim] * det * dx[i];
             }
+        // domain decomposition, src/Cassemble.cpp:906-914: weight 1 / zeta(aT, aT)
+        const double weight = nZeta ? 1. / evaluateZeta(zeta_indices, zeta_indptr, aTdx, aTdx) : 1.;
         for (int a = 0; a < 3 * outdim; a++)
             if (is_DG || vertLabel[aAdx[a / outdim]] > 0) {
 #pragma omp atomic update
-                fd[outdim * aAdx[a / outdim] + a % outdim] += termf[a];
+                fd[outdim * aAdx[a / outdim] + a % outdim] += termf[a] * weight;
             }
     }
     return 0;

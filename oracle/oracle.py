@@ -32,7 +32,8 @@ def load():
     lib.oracle_par_system.restype = C.c_void_p
     lib.oracle_par_system.argtypes = [C.c_int, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int,
                                       _dp, C.c_double, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
-                                      _dp, C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+                                      _dp, C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      _lp, _lp, C.c_long]
     for name in ("oracle_result_nnz", "oracle_result_npairs", "oracle_result_nprobes", "oracle_result_nclose",
                  "oracle_result_nbr_total"):
         getattr(lib, name).restype = C.c_long
@@ -57,7 +58,7 @@ def load():
                                           C.c_double, _dp, _dp, _dp, _dp]
     lib.oracle_quadrule.argtypes = [_dp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
     lib.oracle_par_forcing.argtypes = [C.c_int, _lp, _lp, _dp, _lp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_char_p,
-                                       C.c_int, _dp]
+                                       C.c_int, _dp, _lp, _lp, C.c_long]
     lib.oracle_par_evaluateMass.argtypes = [_dp, _dp, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int]
     _LIB = lib
     return lib
@@ -79,12 +80,24 @@ def _i64(a):
     return np.ascontiguousarray(np.asarray(a).astype(np.int64).ravel())
 
 
+def _zeta(mesh):
+    """Domain-decomposition indicator of the mesh dictionary as the reference's wrapper reads it
+    (cython/nlfem.pyx:205-214): mesh["nZeta"] and the CSR rows of mesh["ZetaIndicator"]."""
+    n = int(mesh.get("nZeta", 0))
+    if n <= 0:
+        z = np.zeros(1, dtype=np.int64)
+        return z, z, 0
+    Z = mesh["ZetaIndicator"]
+    return _i64(Z.indices), _i64(Z.indptr), n
+
+
 def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None):
     """Assemble with the oracle.  Returns a dict with ``A`` (scipy CSR, K x K) and counters.
 
     ``outer=(begin, end)`` restricts the outer-element loop to a sample (for timing)."""
     from nlfem_amd.config import parse
     a = parse(mesh, kernel, conf)
+    zi, zp, nz = _zeta(mesh)
     lib = load()
     Pg = a.Pg if a.Pg.size else np.zeros(1)
     dg = a.dg if a.dg.size else np.zeros(1)
@@ -93,7 +106,7 @@ def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None
                               a.sqdelta, a.is_DG, a.kernel.encode(), a.method_remote.encode(),
                               a.method_close.encode(), a.is_PlacePointOnCap, a.outdim, _d(Pg), a.degree, _d(dg),
                               a.fractional_s, a.is_fullConnectedComponentSearch, int(nthreads), int(keep_neighbours),
-                              int(outer[0]) if outer else 0, int(outer[1]) if outer else 0)
+                              int(outer[0]) if outer else 0, int(outer[1]) if outer else 0, _l(zi), _l(zp), nz)
     if not h:
         raise ValueError("oracle: " + lib.oracle_last_error().decode())
     try:
@@ -246,9 +259,10 @@ def load_vector(mesh, load_fn, conf):
     kern = {"function": "constant", "horizon": 0.1, "outputdim": mesh.get("outdim", 1)}
     a = parse(mesh, kern, conf)
     fd = np.zeros(a.K)
+    zi, zp, nz = _zeta(mesh)
     r = lib.oracle_par_forcing(a.K, _l(a.elements), _l(a.elementLabels), _d(a.vertices), _l(a.vertexLabels), a.nE,
                                _d(a.Px), a.Px.size // 2, _d(a.dx), a.is_DG, load_fn["function"].encode(), a.outdim,
-                               _d(fd))
+                               _d(fd), _l(zi), _l(zp), nz)
     if r:
         raise ValueError("unknown forcing " + load_fn["function"])
     return fd

@@ -104,6 +104,38 @@ def test_parity_tutorial_gmsh_meshes(name, art):
         _check_rigid(name, A, mesh, kernel, conf)
 
 
+@pytest.mark.parametrize("name", ["C1", "C3s", "C5s", "DG"])
+def test_parity_domain_decomposition_weights(name):
+    """nZeta > 0: every pair weighted by 1 / |Z_aT n Z_bT| (src/mathhelpers.cpp:19-38,
+    src/Cassemble.cpp:648-656), on a random cover (every element in subdomain 0, about half in
+    1, a fifth in 2), against the oracle: remote pairs (C1, DG), tensor-Gauss close pairs (C3s)
+    and the field kernel (C5s)."""
+    from test_oracle_zeta import random_cover, with_zeta
+    mesh, kernel, conf = case(name)
+    zmesh = with_zeta(mesh, random_cover(mesh["nE"], seed=11))
+    A, ref = _check_case(f"{name}-zeta", zmesh, kernel, conf)
+    plain = oracle.par_system(mesh, kernel, conf)["A"]
+    assert abs(A - plain).max() > 0.05 * abs(plain).max()  # the weights act
+
+
+def test_domain_decomposition_drop_in_and_load_vector(tmp_path):
+    """The weights through par_assemble's file contract ("system" and "forcing")."""
+    from test_oracle_zeta import random_cover, with_zeta
+    mesh, kernel, conf = case("C2s")
+    zmesh = with_zeta(mesh, random_cover(mesh["nE"], seed=3))
+    conf2 = dict(conf)
+    conf2["savePath"] = str(tmp_path / "A.arma")
+    A = nlfem.stiffnessMatrix(zmesh, kernel, conf2)
+    R = oracle.par_system(zmesh, kernel, conf)["A"]
+    same, rel, rowrel = compare(A, R)
+    print(f"[parity] zeta drop-in C2s pattern_equal={same} max_rowrel={rowrel:.3e}", flush=True)
+    assert same and rowrel <= 1e-12
+    f = nlfem.loadVector(zmesh, {"function": "linear"}, conf)
+    fr = oracle.load_vector(zmesh, {"function": "linear"}, conf)
+    f0 = oracle.load_vector(mesh, {"function": "linear"}, conf)
+    assert np.allclose(f, fr, rtol=1e-13, atol=1e-15) and not np.allclose(f, f0)
+
+
 def test_parity_vertex_of_valence_140():
     """A fan of 140 triangles around one vertex: dual-graph rows of ~140 elements and a node
     row merged from 140 partial rows (more than one merge chunk)."""
