@@ -74,28 +74,69 @@ def measured_traffic(kernel_name, workload):
     return None
 
 
-def cpu_baseline(mesh, kernel, conf, budget_s=15.0):
-    """FP64 OpenMP oracle on a sample of outer elements (centre of the mesh), all host cores."""
+def _cpu_sample(oracle, mesh, kernel, conf, budget_s, nthreads):
+    """Oracle run on every stride-th outer element of the whole mesh (interior and boundary
+    elements in their true proportion), with the stride calibrated so that the run takes about
+    budget_s seconds on nthreads threads (0: all cores)."""
+    nE = mesh["nE"]
+    stride = max(1, nE // 64)
+    while True:
+        r = oracle.par_system(mesh, kernel, conf, nthreads=nthreads, stride=stride)
+        if r["seconds"] > 1.0 or stride == 1:
+            break
+        stride = max(1, stride // 8)
+    target = max(1, int(stride * r["seconds"] / budget_s))
+    if target < stride * 0.7:
+        stride = target
+        r = oracle.par_system(mesh, kernel, conf, nthreads=nthreads, stride=stride)
+    n_outer = (nE + stride - 1) // stride
+    return r, stride, n_outer
+
+
+def cpu_baseline(mesh, kernel, conf, budget_s=15.0, budget_1t_s=8.0):
+    """The FP64 OpenMP oracle (the reference's cost structure: per-thread std::map, queue BFS
+    with the O(nE) visited reset per outer element, every thread scanning all elements) on a
+    strided sample of the whole mesh's outer elements, on all host cores and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     nE = mesh["nE"]
     cores = os.cpu_count() or 1
-    s = max(64, min(nE, 4 * cores))
-    # calibrate, then size the sample to the time budget
-    while True:
-        b = nE // 2 - s // 2
-        r = oracle.par_system(mesh, kernel, conf, outer=(b, b + s))
-        if r["seconds"] > 1.0 or s >= nE:
-            break
-        s = min(nE, s * 4)
-    target = int(s * budget_s / max(r["seconds"], 1e-3))
-    if target > s * 1.5:
-        s = min(nE, target)
-        b = nE // 2 - s // 2
-        r = oracle.par_system(mesh, kernel, conf, outer=(b, b + s))
-    return {"value": r["n_pairs"] / r["seconds"], "unit": "element-pairs/s", "cores": cores, "kind": "port",
-            "sample": f"{s} of {nE} outer elements (centre of the mesh), {r['n_pairs']} pairs, "
-                      f"{r['seconds']:.2f} s incl. dual graph and merge, OMP threads={cores}"}
+    r, stride, n_outer = _cpu_sample(oracle, mesh, kernel, conf, budget_s, 0)
+    out = {"value": r["n_pairs"] / r["seconds"], "unit": "element-pairs/s", "cores": cores, "kind": "port",
+           "sample": f"every {stride}-th of {nE} outer elements ({n_outer}), {r['n_pairs']} pairs, "
+                     f"{r['seconds']:.2f} s incl. dual graph and merge, OMP threads={cores}"}
+    r1, s1, n1 = _cpu_sample(oracle, mesh, kernel, conf, budget_1t_s, 1)
+    out["one_thread"] = {"value": r1["n_pairs"] / r1["seconds"], "unit": "element-pairs/s", "cores": 1,
+                         "sample": f"every {s1}-th of {nE} outer elements ({n1}), {r1['n_pairs']} pairs, "
+                                   f"{r1['seconds']:.2f} s"}
+    return out
+
+
+def end_to_end(mesh, kernel, conf, reps=3):
+    """Host arrays to host CSR, as a caller of stiffnessMatrix sees it: plan creation (checks,
+    estimates, mesh upload H2D, element set-up), the first assembly of the plan (including its
+    device allocations), the CSR copy D2H into pageable host memory, and the plan's teardown.
+    Medians over reps."""
+    from nlfem_amd import nlfem
+    rows = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        a = nlfem.Assembler(mesh, kernel, conf)
+        t1 = time.perf_counter()
+        a.assemble()
+        t2 = time.perf_counter()
+        A = a.csr()
+        t3 = time.perf_counter()
+        a.close()
+        t4 = time.perf_counter()
+        rows.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3, t4 - t0))
+    med = np.median(np.array(rows) * 1000.0, axis=0)
+    h2d = sum(np.asarray(mesh[k]).nbytes for k in ("elements", "elementLabels", "vertices", "vertexLabels"))
+    d2h = A.indptr.size * 8 + A.nnz * (4 + 8)
+    return {"ms_total": round(float(med[4]), 2), "ms_create_h2d": round(float(med[0]), 2),
+            "ms_first_assembly": round(float(med[1]), 2), "ms_csr_d2h": round(float(med[2]), 2),
+            "ms_destroy": round(float(med[3]), 2), "h2d_bytes": int(h2d), "d2h_bytes": int(d2h),
+            "d2h_GBs": round(d2h / (med[2] / 1000.0) / 1e9, 2), "reps": reps}
 
 
 def oracle_matrix(mesh, kernel, conf):
@@ -143,6 +184,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (host to host) timing")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and surrogate+mixed timings of the same workload "
                          "(batch: skip the per-mesh error checks)")
@@ -262,10 +304,13 @@ def main():
                              "peak": 2500.0, "unit": "TFLOP/s",
                              "frac": round(f / (st2["ms_cut"] / 1000) / 2.5e15, 4)}
             out["variants"][name] = v
+    asm.close()
+    if not args.no_e2e:
+        print("[bench] end to end (host arrays to host CSR)", file=sys.stderr, flush=True)
+        out["end_to_end"] = end_to_end(mesh, kernel, conf)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(mesh, kernel, conf, args.cpu_budget)
-    asm.close()
     print(json.dumps(out))
 
 
@@ -296,6 +341,32 @@ def run_batch(args):
     st = b.stats()
     blocks = b.matrices()
     b.close()
+    sur = None
+    if not args.no_variants:
+        # cut pairs through the field kernel's surrogate (default model), against the exact blocks
+        print("[bench] variant surrogate", file=sys.stderr, flush=True)
+        bs = batch.BatchAssembler(meshes, deltas, kernel, conf)
+        bs.asm.set_cut_items(cut_items)
+        bs.set_surrogate("default")
+        for i in range(args.warmup):
+            bs.assemble()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            bs.assemble()
+        ms_sur = (time.perf_counter() - t0) * 1000.0 / args.steps
+        st2 = bs.stats()
+        sblocks = bs.matrices()
+        bs.close()
+        serr = [float(spla.norm(S - B) / spla.norm(B)) for S, B in zip(sblocks, blocks)]
+        num = np.sqrt(sum(spla.norm(S - B) ** 2 for S, B in zip(sblocks, blocks)))
+        den = np.sqrt(sum(spla.norm(B) ** 2 for B in blocks))
+        sur = {"ms_per_step": ms_sur, "value": st2["pairs"] / (ms_sur / 1000.0), "speedup_vs_fp64": ms_step / ms_sur,
+               "ms_cut": round(st2["ms_cut"], 3), "ms_cut_fp64": round(st["ms_cut"], 3),
+               "cut_pairs_by_surrogate": int(st2["sur_pairs"]), "cut_pairs_exact_fallback": int(st2["sur_rejected"]),
+               "network_items": int(st2["sur_items"]),
+               "rel_err_fro_vs_fp64": {"all": float(num / den), "max_per_mesh": max(serr),
+                                       "mean_per_mesh": float(np.mean(serr))}}
+        print(f"[bench] surrogate: {sur}", file=sys.stderr, flush=True)
     errs, oerrs = [], []
     for (m, d), B in zip(zip(meshes, deltas), blocks):
         if args.no_variants:  # timing runs (profiles) skip the per-mesh checks
@@ -327,6 +398,8 @@ def run_batch(args):
     }
     if errs:
         out["per_mesh_rel_err_fro"] = {"max": max(errs), "mean": float(np.mean(errs))}
+    if sur is not None:
+        out["variants"] = {"surrogate": sur}
     if oerrs:
         out["vs_oracle"] = {"meshes": len(oerrs), "oracle_threads": os.cpu_count() or 1,
                             "max_rel_err_fro": max(e["rel_err_fro"] for e in oerrs),

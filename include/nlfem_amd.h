@@ -99,12 +99,16 @@ int nlfem_plan_neighbours(const nlfem_plan* plan, long long* ptr, int* nbr, unsi
  * per-inner-element sum of the cut pairs' staged termLocalPrime, which runs between cut and
  * compact).  n = number of doubles available in out; returns NLFEM_OK. */
 int nlfem_plan_stats(const nlfem_plan* plan, double* out, int n);
-/* Cut pairs through the surrogate network (no counterpart in the reference; constant kernel,
- * retriangulate).  nnet = 1 or 2 networks (params[nnet][9737], norm[nnet][42], see the
- * surrogate section below); nnet = 0 switches it off.  A pair goes to the exact method when
- * one of its items lies outside [smin, smax] (sqrt|det bT| / delta) or the two networks
- * disagree by more than tol on an output ratio.  Stats gain (after the phase times) the
- * pairs taken by the surrogate, the pairs handed back, and the network items evaluated. */
+/* Cut pairs through the surrogate network (no counterpart in the reference; retriangulate).
+ * Constant kernel: nnet = 1 or 2 networks (params[nnet][9737], norm[nnet][42], see the
+ * surrogate section below), the ensemble; a pair goes to the exact method when one of its
+ * items lies outside [smin, smax] (sqrt|det bT| / delta) or the two networks disagree by more
+ * than tol on an output ratio.  Microelastic field kernel (outdim 2, CG): nnet = 2, network 0
+ * and 1 give the first and the last nine of the 18 item outputs; the outputs are corrected to
+ * satisfy the rotation constraints exactly, and a pair goes to the exact method when an item
+ * lies outside [smin, smax] or its correction exceeds tol relative to its outputs.
+ * nnet = 0 switches the surrogate off.  Stats gain (after the phase times) the pairs taken by
+ * the surrogate, the pairs handed back, and the network items evaluated. */
 /* Override the mesh-size estimates (longest edge, smallest and mean element area) from which
  * the BFS and pattern table capacities are derived; for meshes made of several disconnected
  * parts of different resolution, pass the values of the part with the most elements per
@@ -137,8 +141,10 @@ int nlfem_plan_autotune(nlfem_plan* plan, int* chosen, double* ms);
 int nlfem_plan_set_surrogate(nlfem_plan* plan, int nnet, const float* params, const float* norm, float tol,
                              double smin, double smax);
 /* Training samples from the exact FP64 cut-pair sums of an assembled plan: feat[maxn][12],
- * targ[maxn][9] (ratios to the full-triangle sums); items kept when a hash of
- * (aT, bT, k, salt) is a multiple of stride.  *count = qualifying items (may exceed maxn). */
+ * targ[maxn][9] (constant kernel: ratios to the full-triangle sums) or targ[maxn][18]
+ * (microelastic field kernel: the six symmetric tensors sum psi_a psi_b k w of the item in its
+ * frame, in units of c |det bT| / delta); items kept when a hash of (aT, bT, k, salt) is a
+ * multiple of stride.  *count = qualifying items (may exceed maxn). */
 int nlfem_plan_surrogate_samples(nlfem_plan* plan, long long maxn, unsigned stride, unsigned salt, double smin,
                                  double smax, float* feat, float* targ, long long* count);
 void nlfem_plan_destroy(nlfem_plan* plan);

@@ -84,6 +84,12 @@ class BatchAssembler:
         """Pairs per chunk of the cut kernel tuned on this batch (see Assembler.autotune)."""
         return self.asm.autotune()
 
+    def set_surrogate(self, model="default", tol=None, smin=None, smax=None, enabled=True):
+        """Cut pairs through the surrogate (see Assembler.set_surrogate; the batch's horizon is 1
+        after scaling, which is what the networks' delta-normalised inputs expect)."""
+        self.asm.set_surrogate(model, tol, smin, smax, enabled)
+        return self
+
     def stats(self):
         return self.asm.stats()
 

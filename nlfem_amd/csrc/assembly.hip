@@ -1371,8 +1371,9 @@ struct SurArgs {
     const long long* roff;
     int* rcnt;
     int list_in;            // the exact cut kernel reads its pairs from rlist / rcnt
-    float* sfeat;           // training samples [smaxn][12] and targets [smaxn][9]
+    float* sfeat;           // training samples [smaxn][12] and targets [smaxn][ntarg]
     float* starg;
+    int ntarg;              // 9 (scalar constant kernel) or 18 (microelastic field kernel)
     long long smaxn;
     unsigned stride, salt;  // keep an item when hash(aT, bT, k, salt) % stride == 0
 };
@@ -2474,7 +2475,7 @@ static bool cut_items_fit(int nPx, int items) { return nPx <= items && nPx * CUT
 enum SurItem : int { SUR_ZERO = 0, SUR_FULL = 1, SUR_PARTIAL = 2, SUR_OUT = 3 };
 
 NLFEM_DEV int sur_item(double x0, double x1, const double* E, double sqdelta, double idelta, double smin,
-                       double smax, float* f) {
+                       double smax, float* f, double* rot = nullptr) {
     double u[6], r2[3];
     int nin = 0;
     for (int i = 0; i < 3; i++) {
@@ -2508,6 +2509,7 @@ NLFEM_DEV int sur_item(double x0, double x1, const double* E, double sqdelta, do
     const double g0 = (u[0] + u[2] + u[4]) / 3.0, g1 = (u[1] + u[3] + u[5]) / 3.0;
     const double gn = sqrt(g0 * g0 + g1 * g1);
     const double cs = g0 / gn, sn = g1 / gn, is = 1.0 / sz;
+    if (rot) { rot[0] = cs; rot[1] = sn; }
     for (int i = 0; i < 3; i++) {
         f[2 * i] = (float)(cs * u[2 * i] + sn * u[2 * i + 1]);
         f[2 * i + 1] = (float)(-sn * u[2 * i] + cs * u[2 * i + 1]);
@@ -2521,6 +2523,39 @@ NLFEM_DEV unsigned sur_hash(unsigned aT, unsigned b, unsigned k, unsigned salt) 
     unsigned h = aT * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ (k + 1u) * 0xC2B2AE3Du ^ salt;
     h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
     return h;
+}
+
+// A symmetric 2 x 2 tensor (xx, xy, yy) into the surrogate's frame, Q T Q^T with
+// Q = [[cs, sn], [-sn, cs]] (the rotation that puts bT's centroid on +x), and back (Q^T T Q).
+NLFEM_DEV void field_to_frame(double xx, double xy, double yy, double cs, double sn, double* t) {
+    t[0] = cs * cs * xx + 2.0 * cs * sn * xy + sn * sn * yy;
+    t[1] = -cs * sn * xx + (cs * cs - sn * sn) * xy + cs * sn * yy;
+    t[2] = sn * sn * xx - 2.0 * cs * sn * xy + cs * cs * yy;
+}
+NLFEM_DEV void field_from_frame(double xx, double xy, double yy, double cs, double sn, double* t) {
+    field_to_frame(xx, xy, yy, cs, -sn, t);
+}
+// Scale of the field surrogate's outputs: the inner rule's sums over the whole of bT of
+// psi_a psi_b times the trace of the field kernel (c / |x - y|), in units of c |det bT| / delta:
+// F_q = sum_i dy_i psi_a psi_b(P_i) delta / |x - y_i| for the six index pairs q = (a <= b).
+// Every component of a partial item's tensor lies in [-F_q, F_q] (|z_i z_j| <= |z|^2, and the
+// integrands of the trace are positive), so the networks learn ratios in [-1, 1].
+NLFEM_DEV void field_scale(double x0, double x1, const double* E, const QuadArgs& q, double delta, double* F) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) F[i] = 0.0;
+    for (int i = 0; i < q.nPy; i++) {
+        double y[2];
+        toPhys(E, q.Py[2 * i], q.Py[2 * i + 1], y);
+        const double d0 = x0 - y[0], d1 = x1 - y[1];
+        const double w = q.dy[i] * delta / sqrt(d0 * d0 + d1 * d1);
+        const double* ps = &q.psiy[3 * i];
+        F[0] += w * ps[0] * ps[0];
+        F[1] += w * ps[0] * ps[1];
+        F[2] += w * ps[0] * ps[2];
+        F[3] += w * ps[1] * ps[1];
+        F[4] += w * ps[1] * ps[2];
+        F[5] += w * ps[2] * ps[2];
+    }
 }
 
 struct PointCollector {
@@ -2836,6 +2871,36 @@ __global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? 3 : 1) void k_cu
                             for (int q = 0; q < 6; q++) a.sur.starg[idx * 9 + q] = (float)(ia[4 + q] / (cd * a.sur.q2[q]));
                             for (int bb = 0; bb < 3; bb++)
                                 a.sur.starg[idx * 9 + 6 + bb] = (float)(ia[1 + bb] / (cd * a.sur.q1[bb]));
+                        }
+                    }
+                }
+            }
+            if (tid < nit && OD == 2 && KER == KER_MICRO_FIELD) {
+                // field kernel: the six symmetric 2 x 2 tensors sLP_ab = sum psi_a psi_b k(y, x) w in
+                // the network's frame (bT's centroid on +x), as ratios to field_scale's F_q
+                const int p = tid / nPx, k = tid % nPx;
+                const int b = pb_[p];
+                if (sur_hash((unsigned)aT, (unsigned)b, (unsigned)k, a.sur.salt) % a.sur.stride == 0) {
+                    const ElemRec B = a.elem[b];
+                    const double idelta = 1.0 / sqrt(a.kp.sqdelta);
+                    float f[12];
+                    double rot[2];
+                    if (sur_item(xk[2 * k], xk[2 * k + 1], B.E, a.kp.sqdelta, idelta, a.sur.smin, a.sur.smax, f,
+                                 rot) == SUR_PARTIAL) {
+                        const long long idx = (long long)atomicAdd(&a.counters[CNT_SAMPLES], 1ull);
+                        if (idx < a.sur.smaxn) {
+                            const double* ia = iacc + tid * 10 * OO;
+                            const double inorm = 1.0 / (a.kp.c * B.absDet * idelta);
+                            double Fq[6];
+                            field_scale(xk[2 * k], xk[2 * k + 1], B.E, a.q, 1.0 / idelta, Fq);
+                            for (int i = 0; i < 12; i++) a.sur.sfeat[idx * 12 + i] = f[i];
+                            for (int q = 0; q < 6; q++) {
+                                double t[3];
+                                field_to_frame(ia[4 * OO + q * OO + 0], ia[4 * OO + q * OO + 1],
+                                               ia[4 * OO + q * OO + 3], rot[0], rot[1], t);
+                                for (int o = 0; o < 3; o++)
+                                    a.sur.starg[idx * 18 + 3 * q + o] = (float)(t[o] * inorm / Fq[q]);
+                            }
                         }
                     }
                 }
@@ -3335,6 +3400,366 @@ __global__ __launch_bounds__(256) void k_cut_sur(NumArgs a) {
 }
 
 // ======================================================================= close pairs
+// ======================================================================= surrogate, field kernel
+// Cut pairs of the microelastic field kernel (outdim 2) through the surrogate.  One block of 256
+// threads per outer element, chunks of SURF_ITEMS items (pair x outer point):
+//   A  one thread per item: its class (sur_item); a FULL item gets the exact inner sums over bT
+//      (the inner rule on bT, as an interior pair does); a PARTIAL item queues its 12 features.
+//   B  the two networks on tiles of 16 partial items per wavefront: network 0 returns the
+//      tensors sLP_ab of the symmetric index pairs (00, 01, 02), network 1 those of (11, 12, 22),
+//      each as (xx, xy, yy) in the item's frame and as ratios to field_scale's F_q.
+//   C  one thread per partial item: the smallest change of the 18 outputs (least squares) that
+//      makes sum_b sLP_ab R u_b = 0 for every a (u_b: bT's vertices relative to x, R the
+//      rotation by 90 degrees).  Every exact quadrature sum satisfies these six equations
+//      because z^T R z = 0 at every point, and with innerNonloc_a = sum_b sLP_ab and innerLocal
+//      = sum_a innerNonloc_a the assembled matrix annihilates rotations and translations of the
+//      mesh exactly.  The size of that change relative to the outputs is the residual indicator:
+//      above tol (or for an OUT item) the pair goes to the exact kernel.  Then the tensors go
+//      back to physical units and axes.
+//   D  rejected pairs are listed and their items cleared; the outer accumulation (cut_outer),
+//      the scatter (cut_scatter) and termLocal follow k_cut.
+constexpr int SURF_ITEMS = 128;
+
+static size_t surf_lds_bytes(int nPx, int HN) {
+    const int cp = std::min(SURF_ITEMS / std::max(nPx, 1), CUT_MAXPAIRS);
+    return 8 * ((size_t)SURF_ITEMS * 40 + (size_t)cp * 72 + (size_t)nPx * 10) + 10 * (size_t)HN;
+}
+
+// x := G^-1 r for a symmetric positive definite 6 x 6 matrix G (Cholesky, in place).
+NLFEM_DEV void spd6_solve(double (&G)[6][6], double (&r)[6]) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double d = G[j][j];
+#pragma unroll
+        for (int k = 0; k < j; k++) d -= G[j][k] * G[j][k];
+        d = sqrt(fmax(d, 1e-300));
+        G[j][j] = d;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double s = G[i][j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= G[i][k] * G[j][k];
+            G[i][j] = s / d;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s = r[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) s -= G[i][k] * r[k];
+        r[i] = s / G[i][i];
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        double s = r[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; k++) s -= G[k][i] * r[k];
+        r[i] = s / G[i][i];
+    }
+}
+
+// NNET = 2: one network per half; NNET = 4: two per half (networks 2h and 2h + 1 for half h),
+// their mean is used and their largest disagreement on a ratio is the indicator instead.
+template <int KER, int NNET>
+__global__ __launch_bounds__(256) void k_cut_sur_field(NumArgs a) {
+    static_assert(NNET == 2 || NNET == 4, "two halves, one or two networks each");
+    constexpr int NE = NNET / 2;
+    constexpr int OD = 2, OO = 4, NB = 36, N = 6;
+    const int nPx = a.q.nPx;
+    const int CP = min(SURF_ITEMS / nPx, CUT_MAXPAIRS);
+    extern __shared__ double ldsf[];
+    double* iacc = ldsf;                          // [SURF_ITEMS][10 OO]: innerLocal, innerNonloc, sLP
+    double* pacc = iacc + SURF_ITEMS * 10 * OO;   // [CP][2 NB]: termNonloc, termLocalPrime
+    double* tls = pacc + CP * 2 * NB;             // [nPx][OO]
+    double* xk = tls + nPx * OO;                  // [nPx][2]
+    double* wa = xk + 2 * nPx;                    // [nPx][3] |aT| dx_k psi_i(x_k)
+    double* wd = wa + 3 * nPx;                    // [nPx] dx_k |aT|
+    int* hk = (int*)(wd + nPx);                   // [HN]
+    unsigned short* hv = (unsigned short*)(hk + a.HN);
+    __shared__ float feat[SURF_ITEMS][12];
+    __shared__ float outp[SURF_ITEMS][18];
+    __shared__ double rotd[SURF_ITEMS][2];
+    __shared__ float spread[SURF_ITEMS][4];
+    __shared__ short pitem[SURF_ITEMS];  // partial slot -> item
+    __shared__ int pb_[CUT_MAXPAIRS];
+    __shared__ int prej[CUT_MAXPAIRS];
+    __shared__ int npart, nrej, nacc;
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nB = a.cntB[aT];
+    if (nB == 0) {
+        if (tid == 0) a.sur.rcnt[aT] = 0;
+        return;
+    }
+    const double idelta = 1.0 / sqrt(a.kp.sqdelta);
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    for (int i = tid; i < nPx * OO; i += 256) tls[i] = 0.0;
+    if (tid < nPx) {
+        toPhys(A.E, a.q.Px[2 * tid], a.q.Px[2 * tid + 1], &xk[2 * tid]);
+        const double c1 = A.absDet * a.q.dx[tid];
+        for (int i = 0; i < 3; i++) wa[3 * tid + i] = c1 * a.q.psix[3 * tid + i];
+        wd[tid] = a.q.dx[tid] * A.absDet;
+    }
+    if (tid == 0) { nrej = 0; nacc = 0; }
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256) {
+            int s = lds_slot(hk, a.HN, a.col[r0 + i]);
+            if (s >= 0) hv[3 * s + j] = (unsigned short)i;
+            else atomicOr(&a.counters[CNT_ERROR], 16ull);
+        }
+    }
+    unsigned myitems = 0;
+    for (int c0 = 0; c0 < nB; c0 += CP) {
+        const int np = min(CP, nB - c0);
+        const int nit = np * nPx;
+        __syncthreads();
+        for (int i = tid; i < SURF_ITEMS * 10 * OO; i += 256) iacc[i] = 0.0;
+        if (tid == 0) npart = 0;
+        if (tid < np) {
+            pb_[tid] = (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - (c0 + tid)] & IDX_MASK);
+            prej[tid] = 0;
+        }
+        __syncthreads();
+        // A: classes; exact sums of FULL items; features of PARTIAL items
+        if (tid < nit) {
+            const int p = tid / nPx, k = tid % nPx;
+            const ElemRec B = a.elem[pb_[p]];
+            const double x0 = xk[2 * k], x1 = xk[2 * k + 1];
+            float f[12];
+            double rot[2];
+            const int c = sur_item(x0, x1, B.E, a.kp.sqdelta, idelta, a.sur.smin, a.sur.smax, f, rot);
+            if (c == SUR_PARTIAL) {
+                const int sl = atomicAdd(&npart, 1);
+                pitem[sl] = (short)tid;
+#pragma unroll
+                for (int i = 0; i < 12; i++) feat[sl][i] = f[i];
+                rotd[sl][0] = rot[0];
+                rotd[sl][1] = rot[1];
+            } else if (c == SUR_OUT) {
+                prej[p] = 1;
+            } else if (c == SUR_FULL) {
+                double inL[OO], inN[3 * OO], sLP[6 * OO];
+#pragma unroll
+                for (int o = 0; o < OO; o++) inL[o] = 0;
+#pragma unroll
+                for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+#pragma unroll
+                for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
+                for (int i = 0; i < a.q.nPy; i++) {
+                    double y[2];
+                    toPhys(B.E, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                    double K[OO];
+                    kernel_eval<KER, OO, double>(x0 - y[0], x1 - y[1], a.kp, K);
+                    const double psi[3] = {a.q.psiy[3 * i], a.q.psiy[3 * i + 1], a.q.psiy[3 * i + 2]};
+                    inner_point<OD, double, true, true, true, true>(K, a.q.dy[i] * B.absDet, psi, inL, inN, sLP);
+                }
+                double* ia = iacc + tid * 10 * OO;
+#pragma unroll
+                for (int o = 0; o < OO; o++) ia[o] = inL[o];
+#pragma unroll
+                for (int o = 0; o < 3 * OO; o++) ia[OO + o] = inN[o];
+#pragma unroll
+                for (int o = 0; o < 6 * OO; o++) ia[4 * OO + o] = sLP[o];
+            }
+        }
+        __syncthreads();
+        // B: networks 0 and 1 (the two halves of the 18 outputs), 16 items per wavefront
+        const int npi = npart;
+        myitems += (tid == 0) ? npi : 0;
+        {
+            const int j = lane & 15, q = 4 * (lane >> 4);
+            for (int tile = w; tile * 16 < npi; tile += 4) {
+                const int sl = tile * 16 + j;
+                const bool valid = sl < npi;
+                float sp = 0;
+#pragma unroll 1
+                for (int h = 0; h < 2; h++) {
+                    float mo[4] = {0, 0, 0, 0};
+#pragma unroll 1
+                    for (int m = 0; m < NE; m++) {
+                        const int net = h * NE + m;
+                        const float* nm = a.sur.norm + net * mlp::NNORM;
+                        mlp::f32x4 xin = {0, 0, 0, 0};
+                        if (valid && q < mlp::NIN) {
+#pragma unroll
+                            for (int i = 0; i < 4; i++) xin[i] = (feat[sl][q + i] - nm[mlp::NORM_MU + q + i]) * nm[mlp::NORM_ISD + q + i];
+                        }
+                        const mlp::f32x4 y = mlp::forward_tile(a.sur.wf + (size_t)net * mlp::NFRAG_FWD * 64,
+                                                               a.sur.bias + net * mlp::BIAS_PAD, xin, lane);
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int oi = q + i < mlp::NOUT ? q + i : 0;
+                            const float v = y[i] * nm[mlp::NORM_OSD + oi] + nm[mlp::NORM_OMU + oi];
+                            if (m == 0) {
+                                mo[i] = v;
+                            } else {
+                                if (q + i < mlp::NOUT) sp = fmaxf(sp, fabsf(mo[i] - v));
+                                mo[i] = 0.5f * (mo[i] + v);
+                            }
+                        }
+                    }
+                    if (valid) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            if (q + i < mlp::NOUT) outp[sl][9 * h + q + i] = mo[i];
+                    }
+                }
+                if (valid) spread[sl][lane >> 4] = sp;
+            }
+        }
+        __syncthreads();
+        // C: constraint projection, indicator, back to physical units and axes
+        for (int sl = tid; sl < npi; sl += 256) {
+            const int it = pitem[sl], p = it / nPx, k = it % nPx;
+            const ElemRec B = a.elem[pb_[p]];
+            const double cs = rotd[sl][0], sn = rotd[sl][1];
+            double Fq[6];
+            field_scale(xk[2 * k], xk[2 * k + 1], B.E, a.q, 1.0 / idelta, Fq);
+            double s[18];  // in units of c |det bT| / delta
+#pragma unroll
+            for (int i = 0; i < 18; i++) s[i] = (double)outp[sl][i] * Fq[i / 3];
+            // v_b = R u_b, u_b = bT's vertex b relative to x in delta units and frame axes (FP64,
+            // so that the constraints hold for the true geometry)
+            double vx[3], vy[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const double d0 = (B.E[2 * b] - xk[2 * k]) * idelta, d1 = (B.E[2 * b + 1] - xk[2 * k + 1]) * idelta;
+                const double u0 = cs * d0 + sn * d1, u1 = -sn * d0 + cs * d1;
+                vx[b] = -u1;
+                vy[b] = u0;
+            }
+            // constraint rows (a, x) = sum_b S_ab,xx v_b,x + S_ab,xy v_b,y;
+            //                 (a, y) = sum_b S_ab,xy v_b,x + S_ab,yy v_b,y
+            double C[6][18];
+#pragma unroll
+            for (int r = 0; r < 6; r++)
+#pragma unroll
+                for (int c = 0; c < 18; c++) C[r][c] = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < 3; aa++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const int qq = sym6(aa, b);
+                    C[2 * aa][3 * qq] += vx[b];
+                    C[2 * aa][3 * qq + 1] += vy[b];
+                    C[2 * aa + 1][3 * qq + 1] += vx[b];
+                    C[2 * aa + 1][3 * qq + 2] += vy[b];
+                }
+            double r[6], G[6][6];
+            double tr = 0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                double v = 0;
+#pragma unroll
+                for (int c = 0; c < 18; c++) v += C[i][c] * s[c];
+                r[i] = v;
+#pragma unroll
+                for (int j2 = 0; j2 < 6; j2++) {
+                    double g = 0;
+#pragma unroll
+                    for (int c = 0; c < 18; c++) g += C[i][c] * C[j2][c];
+                    G[i][j2] = g;
+                }
+                tr += G[i][i];
+            }
+#pragma unroll
+            for (int i = 0; i < 6; i++) G[i][i] += 1e-12 * tr;
+            spd6_solve(G, r);
+            double dmax = 0, smax = 0;
+#pragma unroll
+            for (int c = 0; c < 18; c++) {
+                double d = 0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) d += C[i][c] * r[i];
+                smax = fmax(smax, fabs(s[c]));
+                s[c] -= d;
+                dmax = fmax(dmax, fabs(d));
+            }
+            if constexpr (NE == 2) {
+                const float sp = fmaxf(fmaxf(spread[sl][0], spread[sl][1]), fmaxf(spread[sl][2], spread[sl][3]));
+                if (!(sp <= a.sur.tol)) prej[p] = 1;
+            } else {
+                if (!(dmax <= a.sur.tol * fmax(smax, 1e-30))) prej[p] = 1;
+            }
+            const double sc = a.kp.c * B.absDet * idelta;
+            double* ia = iacc + it * 10 * OO;
+            double sLP[6][OO];
+#pragma unroll
+            for (int qq = 0; qq < 6; qq++) {
+                double t[3];
+                field_from_frame(s[3 * qq], s[3 * qq + 1], s[3 * qq + 2], cs, sn, t);
+                sLP[qq][0] = t[0] * sc;
+                sLP[qq][1] = t[1] * sc;
+                sLP[qq][2] = t[1] * sc;
+                sLP[qq][3] = t[2] * sc;
+            }
+#pragma unroll
+            for (int qq = 0; qq < 6; qq++)
+#pragma unroll
+                for (int o = 0; o < OO; o++) ia[4 * OO + qq * OO + o] = sLP[qq][o];
+            double inL[OO] = {0, 0, 0, 0};
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int o = 0; o < OO; o++) {
+                    const double v = sLP[sym6(0, b)][o] + sLP[sym6(1, b)][o] + sLP[sym6(2, b)][o];
+                    ia[OO + b * OO + o] = v;
+                    inL[o] += v;
+                }
+#pragma unroll
+            for (int o = 0; o < OO; o++) ia[o] = inL[o];
+        }
+        __syncthreads();
+        // D: rejected pairs go to the exact kernel; their items are cleared
+        if (tid < np) {
+            if (prej[tid]) {
+                const int rr = atomicAdd(&nrej, 1);
+                a.sur.rlist[a.sur.roff[aT] + rr] = pb_[tid];
+            } else {
+                atomicAdd(&nacc, 1);
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < nit * 10 * OO; t += 256)
+            if (prej[t / (10 * OO * nPx)]) iacc[t] = 0.0;
+        __syncthreads();
+        if (tid < np && prej[tid]) pb_[tid] = -1;
+        cut_outer<OD, true>(a, np, wa, wd, iacc, pacc, tls);
+        __syncthreads();
+        cut_scatter<OD>(a, np, pb_, pacc, rA, labA, hk, hv);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        a.sur.rcnt[aT] = nrej;
+        atomicAdd(&a.counters[CNT_SUR_ACC], (unsigned long long)nacc);
+        atomicAdd(&a.counters[CNT_SUR_REJ], (unsigned long long)nrej);
+        atomicAdd(&a.counters[CNT_SUR_ITEMS], (unsigned long long)myitems);
+    }
+    // termLocal[N ra + cb] = sum_k |aT| dx_k psi_a(x_k) psi_b(x_k) innerLocal_k (accepted pairs)
+    for (int t = tid; t < NB; t += 256) {
+        const int ra = t / N, cb = t % N;
+        const int i = ra / OD, ao = ra % OD, j = cb / OD, bo = cb % OD;
+        if (!labA[i]) continue;
+        double v = 0;  // (cut_outer stored component 1 in slot 2 as well)
+        for (int k = 0; k < nPx; k++)
+            v += A.absDet * a.q.dx[k] * a.q.psix[3 * k + i] * a.q.psix[3 * k + j] * tls[k * OO + OD * ao + bo];
+        const long long p = a.posTT[9 * aT + 3 * i + j];
+        atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
+}
+
 struct CloseArgs {
     int is_DG, nPg, od;
     const ElemRec* elem;
@@ -3759,8 +4184,11 @@ static NumArgs base_numargs(const Plan& P) {
 // and the 7-point outer rule it was trained with.
 static bool surrogate_applies(const Plan& P) {
     const PlanConfig& c = P.cfg;
-    return P.sur.nnet > 0 && c.kernel == KER_CONSTANT && c.outdim == 1 && c.method_remote == M_RETRIANGULATE &&
-           !c.singular && P.stats.bary == 0 && c.q.nPx <= SUR_ITEMS / 4 && !P.d_zptr;
+    if (P.sur.nnet == 0 || c.method_remote != M_RETRIANGULATE || P.stats.bary > 0 || P.d_zptr) return false;
+    if (c.kernel == KER_CONSTANT && c.outdim == 1) return !c.singular && c.q.nPx <= SUR_ITEMS / 4;
+    // field kernel: the two networks are the two halves of the 18 outputs
+    return c.kernel == KER_MICRO_FIELD && c.outdim == 2 && (P.sur.nnet == 2 || P.sur.nnet == 4) && !c.is_DG &&
+           c.q.nPx <= SURF_ITEMS && c.q.nPx <= CUT_ITEMS && c.q.nPx * CUT_MAXPAIRS >= CUT_ITEMS;
 }
 
 static void set_surrogate_args(const Plan& P, SurArgs& s) {
@@ -4238,7 +4666,17 @@ void plan_assemble(Plan& P, const Estimates& est) {
             ns.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
             set_surrogate_args(P, ns.sur);
             const size_t ls = (size_t)ns.HN * 16;
-            if (P.sur.nnet == 2) {
+            if (c.kernel == KER_MICRO_FIELD) {
+                const size_t lf = surf_lds_bytes(c.q.nPx, ns.HN);
+                if (lf > 160 * 1024 || P.maxRow > 65535) throw std::length_error("surrogate LDS exceeds 160 KiB");
+                if (P.sur.nnet == 4) {
+                    if (lf > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur_field<KER_MICRO_FIELD, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf);
+                    k_cut_sur_field<KER_MICRO_FIELD, 4><<<c.nE, 256, lf, st>>>(ns);
+                } else {
+                    if (lf > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur_field<KER_MICRO_FIELD, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf);
+                    k_cut_sur_field<KER_MICRO_FIELD, 2><<<c.nE, 256, lf, st>>>(ns);
+                }
+            } else if (P.sur.nnet == 2) {
                 if (ls > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur<KER_CONSTANT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
                 k_cut_sur<KER_CONSTANT, 2><<<c.nE, 256, ls, st>>>(ns);
             } else {
@@ -4373,14 +4811,15 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
 void plan_set_surrogate(Plan& P, int nnet, const float* params, const float* norm, float tol, double smin,
                         double smax) {
-    if (nnet < 0 || nnet > 2) throw std::invalid_argument("the surrogate ensemble has one or two networks");
+    if (nnet < 0 || nnet > 4 || (nnet > 2 && P.cfg.kernel != KER_MICRO_FIELD) || (nnet == 3))
+        throw std::invalid_argument("the surrogate has one or two networks (field kernel: two or four)");
     if (nnet > 0 && (!params || !norm)) throw std::invalid_argument("surrogate parameters missing");
     if (nnet > 0 && !(tol >= 0)) throw std::invalid_argument("surrogate tolerance must be >= 0");
     if (nnet > 0 && !(smin >= 0 && smax > smin)) throw std::invalid_argument("invalid surrogate domain");
     if (!P.sur.d_wf) {
-        HIPCHECK(hipMalloc((void**)&P.sur.d_wf, 2 * mlp::NFRAG_FWD * 256 * sizeof(uint16_t)));
-        HIPCHECK(hipMalloc((void**)&P.sur.d_bias, 2 * mlp::BIAS_PAD * sizeof(float)));
-        HIPCHECK(hipMalloc((void**)&P.sur.d_norm, 2 * mlp::NNORM * sizeof(float)));
+        HIPCHECK(hipMalloc((void**)&P.sur.d_wf, 4 * mlp::NFRAG_FWD * 256 * sizeof(uint16_t)));
+        HIPCHECK(hipMalloc((void**)&P.sur.d_bias, 4 * mlp::BIAS_PAD * sizeof(float)));
+        HIPCHECK(hipMalloc((void**)&P.sur.d_norm, 4 * mlp::NNORM * sizeof(float)));
     }
     for (int n = 0; n < nnet; n++)
         mlp::upload(params + (size_t)n * mlp::NPARAM, norm + (size_t)n * mlp::NNORM,
@@ -4400,8 +4839,12 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
                                  float* feat, float* targ) {
     const PlanConfig& c = P.cfg;
     if (!P.assembled) throw std::invalid_argument("assemble the plan before drawing surrogate samples");
-    if (c.kernel != KER_CONSTANT || c.outdim != 1 || c.method_remote != M_RETRIANGULATE || P.stats.bary > 0)
-        throw std::invalid_argument("surrogate samples need the constant kernel with the retriangulate method");
+    const bool field = c.kernel == KER_MICRO_FIELD && c.outdim == 2 && !c.is_DG;
+    if (!((c.kernel == KER_CONSTANT && c.outdim == 1) || field) || c.method_remote != M_RETRIANGULATE ||
+        P.stats.bary > 0)
+        throw std::invalid_argument(
+            "surrogate samples need the constant or the microelastic field kernel (CG) with the retriangulate method");
+    const int NT = field ? 18 : 9;
     if (c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS)
         throw std::invalid_argument("outer rule size not supported by the cut kernel");
     if (stride == 0 || maxn < 0 || (maxn > 0 && (!feat || !targ)) || !(smax > smin))
@@ -4409,7 +4852,7 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
     const hipStream_t st = P.stream;
     float *dF = nullptr, *dT = nullptr;
     HIPCHECK(hipMalloc((void**)&dF, (size_t)std::max<long long>(maxn, 1) * 12 * sizeof(float)));
-    HIPCHECK(hipMalloc((void**)&dT, (size_t)std::max<long long>(maxn, 1) * 9 * sizeof(float)));
+    HIPCHECK(hipMalloc((void**)&dT, (size_t)std::max<long long>(maxn, 1) * NT * sizeof(float)));
     NumArgs nc = base_numargs(P);
     nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
     nc.sur.sfeat = dF;
@@ -4422,20 +4865,26 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
     set_surrogate_args(P, nc.sur);
     nc.sur.sfeat = dF; nc.sur.starg = dT; nc.sur.smaxn = maxn; nc.sur.stride = stride; nc.sur.salt = salt;
     nc.sur.smin = smin; nc.sur.smax = smax;
-    const size_t lb = cut_lds_bytes(c.q.nPx, 1, nc.HN);
+    nc.sur.ntarg = NT;
+    const size_t lb = cut_lds_bytes(c.q.nPx, P.oo, nc.HN);
     if (lb > 160 * 1024 || P.maxRow > 65535) throw std::length_error("cut-pair LDS exceeds 160 KiB");
     unsigned long long h[16];
     try {
         HIPCHECK(hipMemsetAsync(P.d_counters + CNT_SAMPLES, 0, sizeof(unsigned long long), st));
-        if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER_CONSTANT, 1, double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-        k_cut<KER_CONSTANT, 1, double, 1><<<c.nE, 256, lb, st>>>(nc);
+        if (field) {
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER_MICRO_FIELD, 2, double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+            k_cut<KER_MICRO_FIELD, 2, double, 1><<<c.nE, 256, lb, st>>>(nc);
+        } else {
+            if (lb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut<KER_CONSTANT, 1, double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+            k_cut<KER_CONSTANT, 1, double, 1><<<c.nE, 256, lb, st>>>(nc);
+        }
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
         const long long n = (long long)h[CNT_SAMPLES];
         const long long m = std::min(n, maxn);
         if (m > 0) {
             HIPCHECK(hipMemcpy(feat, dF, (size_t)m * 12 * sizeof(float), hipMemcpyDeviceToHost));
-            HIPCHECK(hipMemcpy(targ, dT, (size_t)m * 9 * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(targ, dT, (size_t)m * NT * sizeof(float), hipMemcpyDeviceToHost));
         }
         (void)hipFree(dF);
         (void)hipFree(dT);

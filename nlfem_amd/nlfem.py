@@ -148,7 +148,7 @@ class Assembler:
         if not enabled:
             check(lib().nlfem_plan_set_surrogate(self._h, 0, None, None, 0.0, 0.0, 1.0))
             return self
-        params, norm, meta = surrogate.load(model)
+        params, norm, meta = surrogate.load(model, kernel=self.args.kernel)
         tol = float(meta.get("tol", 1e-3) if tol is None else tol)
         smin = float(meta.get("smin", 0.0) if smin is None else smin)
         smax = float(meta.get("smax", 1.0) if smax is None else smax)
@@ -159,10 +159,11 @@ class Assembler:
         return self
 
     def surrogate_samples(self, maxn, stride=1, salt=0, smin=0.0, smax=1.0):
-        """Training samples (features [n,12], targets [n,9]) from the exact FP64 cut-pair sums of
-        the assembled plan, and the number of qualifying items."""
+        """Training samples (features [n,12], targets [n,9], or [n,18] for the microelastic field
+        kernel) from the exact FP64 cut-pair sums of the assembled plan, and the number of
+        qualifying items."""
         F = np.zeros((max(maxn, 1), 12), dtype=np.float32)
-        T = np.zeros((max(maxn, 1), 9), dtype=np.float32)
+        T = np.zeros((max(maxn, 1), 18 if self.args.outdim == 2 else 9), dtype=np.float32)
         cnt = C.c_longlong()
         check(lib().nlfem_plan_surrogate_samples(self._h, int(maxn), int(stride), int(salt) & 0xFFFFFFFF,
                                                  float(smin), float(smax), F.ctypes.data_as(_fp),

@@ -714,7 +714,7 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
                                 int caps, int outdim, const double* Pg, int degree, const double* dg,
                                 double fractional_s, int fullCC, int nthreads, int keep_neighbours,
                                 int outer_begin, int outer_end, const long* zeta_indices, const long* zeta_indptr,
-                                long nZeta) {
+                                long nZeta, int outer_stride) {
     Mesh m;
     m.nE = nE; m.nV = nV; m.outdim = outdim; m.K = K; m.is_DG = is_DG;
     m.tri = tri; m.triLabel = triLabel; m.verts = verts; m.vertLabel = vertLabel;
@@ -771,6 +771,8 @@ OracleResult* oracle_par_system(int K, const long* tri, const long* triLabel, co
         for (int aTdx = 0; aTdx < nE; aTdx++) {  // every thread scans all elements (src/Cassemble.cpp:555-559)
             // optional sample of the outer loop [outer_begin, outer_end) for timing (CPU baseline)
             if (outer_end > outer_begin && (aTdx < outer_begin || aTdx >= outer_end)) continue;
+            // optional strided sample (every outer_stride-th element of the whole mesh)
+            if (outer_stride > 1 && aTdx % outer_stride != 0) continue;
             const bool mine = (aTdx >= lo && aTdx < hi);
             if (!(mine && triLabel[aTdx])) continue;
             const long* aAdx;

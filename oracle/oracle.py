@@ -33,7 +33,7 @@ def load():
     lib.oracle_par_system.argtypes = [C.c_int, _lp, _lp, _dp, _lp, C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int,
                                       _dp, C.c_double, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
                                       _dp, C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                      _lp, _lp, C.c_long]
+                                      _lp, _lp, C.c_long, C.c_int]
     for name in ("oracle_result_nnz", "oracle_result_npairs", "oracle_result_nprobes", "oracle_result_nclose",
                  "oracle_result_nbr_total"):
         getattr(lib, name).restype = C.c_long
@@ -91,10 +91,11 @@ def _zeta(mesh):
     return _i64(Z.indices), _i64(Z.indptr), n
 
 
-def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None):
+def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None, stride=1):
     """Assemble with the oracle.  Returns a dict with ``A`` (scipy CSR, K x K) and counters.
 
-    ``outer=(begin, end)`` restricts the outer-element loop to a sample (for timing)."""
+    ``outer=(begin, end)`` restricts the outer-element loop to a range and ``stride`` to every
+    stride-th outer element (samples for timing)."""
     from nlfem_amd.config import parse
     a = parse(mesh, kernel, conf)
     zi, zp, nz = _zeta(mesh)
@@ -106,7 +107,8 @@ def par_system(mesh, kernel, conf, nthreads=0, keep_neighbours=False, outer=None
                               a.sqdelta, a.is_DG, a.kernel.encode(), a.method_remote.encode(),
                               a.method_close.encode(), a.is_PlacePointOnCap, a.outdim, _d(Pg), a.degree, _d(dg),
                               a.fractional_s, a.is_fullConnectedComponentSearch, int(nthreads), int(keep_neighbours),
-                              int(outer[0]) if outer else 0, int(outer[1]) if outer else 0, _l(zi), _l(zp), nz)
+                              int(outer[0]) if outer else 0, int(outer[1]) if outer else 0, _l(zi), _l(zp), nz,
+                              int(stride))
     if not h:
         raise ValueError("oracle: " + lib.oracle_last_error().decode())
     try:
