@@ -1064,13 +1064,12 @@ struct PatArgs {
 // row's slot of tmp (RC entries per row) and the row length to rowcnt.
 // BM: the element set is a bitmap over an index window around the row's first element
 // (EH words; an element outside it reports overflow 8 and the host reruns with the hash).
+// pattern_row builds row r with its tables eh (EH ints) and dh (DH ints, then EL ints in bitmap
+// mode) in LDS (k_pattern) or, for rows too large for LDS, in a global-memory slice per block
+// (k_pattern_g, a grid of a few blocks per CU looping over the rows).
 template <bool BM>
-__global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
-    extern __shared__ int lds[];
-    int* eh = lds;
-    int* dh = eh + a.EH;
+NLFEM_DEV void pattern_row(const PatArgs& a, const int r, int* eh, int* dh) {
     __shared__ int nDof, ovf;
-    const int r = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     for (int i = tid; i < a.EH; i += 256) eh[i] = BM ? 0 : -1;
@@ -1233,6 +1232,21 @@ __global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
             a.rowcnt[r] = n;
             atomicMax(&a.counters[CNT_MAXROW], (unsigned long long)n);
         }
+    }
+}
+
+template <bool BM>
+__global__ __launch_bounds__(256) void k_pattern(PatArgs a) {
+    extern __shared__ int lds[];
+    pattern_row<BM>(a, blockIdx.x, lds, lds + a.EH);
+}
+
+// Hashed element and node sets in global memory: block b owns the slice gtab[b (EH + DH) ..).
+__global__ __launch_bounds__(256) void k_pattern_g(PatArgs a, int* gtab) {
+    int* eh = gtab + (size_t)blockIdx.x * (size_t)(a.EH + a.DH);
+    for (int r = blockIdx.x; r < a.nNodes; r += gridDim.x) {
+        pattern_row<false>(a, r, eh, eh + a.EH);
+        __syncthreads();
     }
 }
 
@@ -4185,10 +4199,14 @@ static NumArgs base_numargs(const Plan& P) {
 static bool surrogate_applies(const Plan& P) {
     const PlanConfig& c = P.cfg;
     if (P.sur.nnet == 0 || c.method_remote != M_RETRIANGULATE || P.stats.bary > 0 || P.d_zptr) return false;
-    if (c.kernel == KER_CONSTANT && c.outdim == 1) return !c.singular && c.q.nPx <= SUR_ITEMS / 4;
+    const long long HN = next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+    if (c.kernel == KER_CONSTANT && c.outdim == 1)
+        return !c.singular && c.q.nPx <= SUR_ITEMS / 4 && HN * 16 <= 160 * 1024 && HN * 28 <= 160 * 1024;
     // field kernel: the two networks are the two halves of the 18 outputs
     return c.kernel == KER_MICRO_FIELD && c.outdim == 2 && (P.sur.nnet == 2 || P.sur.nnet == 4) && !c.is_DG &&
-           c.q.nPx <= SURF_ITEMS && c.q.nPx <= CUT_ITEMS && c.q.nPx * CUT_MAXPAIRS >= CUT_ITEMS;
+           c.q.nPx <= SURF_ITEMS && c.q.nPx <= CUT_ITEMS && c.q.nPx * CUT_MAXPAIRS >= CUT_ITEMS &&
+           surf_lds_bytes(c.q.nPx, (int)HN) <= 160 * 1024 && cut_lds_bytes(c.q.nPx, 4, (int)HN, CUT_ITEMS) <= 160 * 1024 &&
+           P.maxRow <= 65535;
 }
 
 static void set_surrogate_args(const Plan& P, SurArgs& s) {
@@ -4451,9 +4469,17 @@ void plan_assemble(Plan& P, const Estimates& est) {
         pa.rowptr = nullptr;
         pa.col = nullptr;
         size_t ldsb = sizeof(int) * (size_t)(pa.EH + pa.DH + pa.EL);
-        if (ldsb > 160 * 1024) throw std::length_error("pattern LDS estimate too large");
+        bool glob = false;  // rows too large for LDS tables: hashed sets in global memory
+        if (ldsb > 160 * 1024) {
+            if (pat_bm) { pat_bm = false; P.pat_nobm = true; continue; }
+            glob = true;
+        }
         HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
-        if (pat_bm) {
+        if (glob) {
+            const int G = (int)std::min<long long>(P.nNodes, 1024);
+            dreserve(P, P.d_gtab, (size_t)G * (size_t)(pa.EH + pa.DH));
+            k_pattern_g<<<G, 256, 0, st>>>(pa, P.d_gtab);
+        } else if (pat_bm) {
             if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_pattern<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
             k_pattern<true><<<P.nNodes, 256, ldsb, st>>>(pa);
         } else {
@@ -4511,6 +4537,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
     na.HN = (HN * 16 <= 64 * 1024) ? (int)HN : 0;
     const size_t nlds = na.HN > 0 ? (size_t)na.HN * 16 : 0;
 
+#define LAUNCH_PAIRS(KER, OD, SEG)                                                                          \
+    do {                                                                                                    \
+        if (c.precision == 1) k_pairs<KER, OD, SEG, float><<<c.nE, 256, nlds, st>>>(na);                    \
+        else k_pairs<KER, OD, SEG, double><<<c.nE, 256, nlds, st>>>(na);                                    \
+    } while (0)
+#define DISPATCH_PAIRS(SEG)                                                        \
+    do {                                                                           \
+        switch (c.kernel) {                                                        \
+            case KER_CONSTANT: LAUNCH_PAIRS(KER_CONSTANT, 1, SEG); break;          \
+            case KER_PARABOLA: LAUNCH_PAIRS(KER_PARABOLA, 1, SEG); break;          \
+            case KER_FRACTIONAL: LAUNCH_PAIRS(KER_FRACTIONAL, 1, SEG); break;      \
+            case KER_MICRO: LAUNCH_PAIRS(KER_MICRO, 1, SEG); break;                \
+            default: LAUNCH_PAIRS(KER_MICRO_FIELD, 2, SEG); break;                 \
+        }                                                                          \
+    } while (0)
     // With domain-decomposition weights every pair is integrated and scattered by the per-pair
     // kernel, its four blocks scaled by the pair's weight (FP64 atomics into val).
 #define LAUNCH_PAIRS_Z(KER, OD, SEG)                                                                        \
@@ -4529,10 +4570,17 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                          \
     } while (0)
 
-    // interior pairs: owner computes into partial rows, then merge (plain stores into val)
+    // interior pairs: owner computes into partial rows, then merge (plain stores into val).
+    // Horizons whose rows outgrow the LDS column table (about 36 elements per horizon for scalar
+    // kernels, 16 for outdim 2) take the per-pair kernel with global atomics instead.
     ev_begin(P, PH_INTERIOR);
+    const bool int_fits = (size_t)((((long long)(1.3 * std::max(P.maxRow, 1)) + 63) / 64 * 64)) * (24 * P.oo + 4) <=
+                          160 * 1024;
     if (P.d_zptr) {
         DISPATCH_PAIRS_Z(0);
+        HIPCHECK(hipGetLastError());
+    } else if (!int_fits) {
+        DISPATCH_PAIRS(0);
         HIPCHECK(hipGetLastError());
     } else {
         int*& d_plen = P.d_wplen;
@@ -4624,29 +4672,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ev_end(P, PH_CLOSE);
     }
 
-#define LAUNCH_PAIRS(KER, OD, SEG)                                                                          \
-    do {                                                                                                    \
-        if (c.precision == 1) k_pairs<KER, OD, SEG, float><<<c.nE, 256, nlds, st>>>(na);                    \
-        else k_pairs<KER, OD, SEG, double><<<c.nE, 256, nlds, st>>>(na);                                    \
-    } while (0)
-#define DISPATCH_PAIRS(SEG)                                                        \
-    do {                                                                           \
-        switch (c.kernel) {                                                        \
-            case KER_CONSTANT: LAUNCH_PAIRS(KER_CONSTANT, 1, SEG); break;          \
-            case KER_PARABOLA: LAUNCH_PAIRS(KER_PARABOLA, 1, SEG); break;          \
-            case KER_FRACTIONAL: LAUNCH_PAIRS(KER_FRACTIONAL, 1, SEG); break;      \
-            case KER_MICRO: LAUNCH_PAIRS(KER_MICRO, 1, SEG); break;                \
-            default: LAUNCH_PAIRS(KER_MICRO_FIELD, 2, SEG); break;                 \
-        }                                                                          \
-    } while (0)
     ev_begin(P, PH_CUT);
     P.cut_items_used = 0;
     bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
+    // LDS of the cut kernels: the column map of aT's rows (k_cut8: 28 B per slot; k_cut: its
+    // chunk buffers and 10 B per slot); rows beyond it (very large horizons) take the per-pair
+    // kernel, whose scatter searches the rows in global memory
+    const int cut_HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+    const bool use_cut8 = c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG;
+    const bool cut_fits = use_cut8 ? (size_t)cut_HN * 28 <= 160 * 1024
+                                   : cut_lds_bytes(c.q.nPx, P.oo, cut_HN, CUT_ITEMS) <= 160 * 1024 && P.maxRow <= 65535;
     if (P.d_zptr) {
         DISPATCH_PAIRS_Z(1);
-    } else if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS) {
+    } else if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS || !cut_fits) {
         DISPATCH_PAIRS(1);
     } else {
         NumArgs nc = na;
@@ -4668,7 +4708,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
             const size_t ls = (size_t)ns.HN * 16;
             if (c.kernel == KER_MICRO_FIELD) {
                 const size_t lf = surf_lds_bytes(c.q.nPx, ns.HN);
-                if (lf > 160 * 1024 || P.maxRow > 65535) throw std::length_error("surrogate LDS exceeds 160 KiB");
+                if (lf > 160 * 1024 || P.maxRow > 65535) throw std::length_error("surrogate LDS exceeds 160 KiB");  // unreachable: see surrogate_applies
                 if (P.sur.nnet == 4) {
                     if (lf > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cut_sur_field<KER_MICRO_FIELD, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf);
                     k_cut_sur_field<KER_MICRO_FIELD, 4><<<c.nE, 256, lf, st>>>(ns);
@@ -4687,7 +4727,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
             nc.sur = ns.sur;
             nc.sur.list_in = 1;
         }
-        nc.HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
+        nc.HN = cut_HN;
         // items per chunk of k_cut: the plan's choice when the outer rule and the LDS allow it
         int items = CUT_ITEMS;
         if (cut_items_fit(c.q.nPx, P.cut_items) && cut_lds_bytes(c.q.nPx, P.oo, nc.HN, P.cut_items) <= 160 * 1024)
@@ -4723,7 +4763,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         }                                                                                                 \
     } while (0)
 #define LAUNCH_CUT8(KER) LAUNCH_CUT8_K(k_cut8, KER)
-        if (c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG) {
+        if (use_cut8) {
             long long ncut = 0;
             HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
@@ -4935,7 +4975,7 @@ void plan_free(Plan& P) {
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
                     P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF, P.d_zptr,
-                    P.d_zidx};
+                    P.d_zidx, P.d_gtab};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     void* wp[] = {P.d_wcnt, P.d_wrowcnt, P.d_wptmp, P.d_wrowOf, P.d_wplen, P.d_wocnt};

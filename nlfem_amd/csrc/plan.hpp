@@ -137,6 +137,7 @@ struct Plan {
     int* d_wrowOf = nullptr;   // row of each node-CSR entry
     int* d_wplen = nullptr;    // partial-row lengths
     int* d_wocnt = nullptr;    // output row lengths
+    int* d_gtab = nullptr;     // pattern hash tables in global memory (rows too large for LDS)
     // allocated capacity (elements) of each device buffer above, keyed by the member's address:
     // repeated assemblies reuse buffers that are large enough
     std::unordered_map<const void*, size_t> dcap;

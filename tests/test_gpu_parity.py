@@ -136,6 +136,28 @@ def test_domain_decomposition_drop_in_and_load_vector(tmp_path):
     assert np.allclose(f, fr, rtol=1e-13, atol=1e-15) and not np.allclose(f, f0)
 
 
+@pytest.mark.parametrize("which", ["scalar30h", "field20h"])
+def test_parity_large_horizons(which):
+    """Horizons whose rows outgrow the LDS tables: 30 elements per horizon with the constant
+    kernel (pattern element and node sets in global memory, cut pairs through the per-pair
+    kernel), 20 with the field kernel (interior pairs through the per-pair kernel)."""
+    from nlfem_amd.mesh import unit_square
+    from nlfem_amd.config import make_configuration
+    if which == "scalar30h":
+        h = 1 / 32
+        mesh = unit_square(64, 30 * h, jitter=0.2, seed=21, hi=2.0)
+        kern = {"function": "constant", "horizon": 30 * h, "outputdim": 1}
+        conf = make_configuration()
+    else:
+        h = 1 / 32
+        mesh = unit_square(48, 20 * h, jitter=0.2, seed=22, hi=1.5, outdim=2)
+        kern = {"function": "linearPrototypeMicroelasticField", "horizon": 20 * h, "outputdim": 2, "fractional_s": -0.5}
+        conf = make_configuration(close="retriangulate", outer="16point", inner="7point")
+    A, ref = _check_case(which, mesh, kern, conf)
+    if kern["outputdim"] == 2:
+        _check_rigid(which, A, mesh, kern, conf)
+
+
 def test_parity_vertex_of_valence_140():
     """A fan of 140 triangles around one vertex: dual-graph rows of ~140 elements and a node
     row merged from 140 partial rows (more than one merge chunk)."""
