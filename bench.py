@@ -185,6 +185,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (host to host) timing")
+    ap.add_argument("--no-checks", action="store_true",
+                    help="batch: skip the per-mesh error checks against single-mesh assemblies (profiling runs)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and surrogate+mixed timings of the same workload "
                          "(batch: skip the per-mesh error checks)")
@@ -369,7 +371,7 @@ def run_batch(args):
         print(f"[bench] surrogate: {sur}", file=sys.stderr, flush=True)
     errs, oerrs = [], []
     for (m, d), B in zip(zip(meshes, deltas), blocks):
-        if args.no_variants:  # timing runs (profiles) skip the per-mesh checks
+        if args.no_variants or args.no_checks:  # timing runs (profiles) skip the per-mesh checks
             break
         k = dict(kernel)
         k["horizon"] = d
