@@ -1,12 +1,15 @@
 // C ABI of libnlfem_amd.so (include/nlfem_amd.h) and the C++ drop-in of the reference's
 // Cassemble.h entry points (include/Cassemble.h of this repository).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nlfem_amd.h"
@@ -231,14 +234,97 @@ extern "C" int nlfem_plan_csr_info(const nlfem_plan* plan, long long* K, long lo
     return NLFEM_OK;
 }
 
+// Device-to-host copy of several device ranges into caller memory that may be pageable.
+// A DMA into pageable memory runs at a few GB/s; instead the ranges stream in chunks through
+// two pinned staging buffers (allocated once per process): while the DMA of chunk i + 1 fills
+// one buffer, host threads copy chunk i out of the other.  Destinations that are pinned
+// already are copied directly.
+struct D2HSeg {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+static int copy_d2h_staged(hipStream_t st, const D2HSeg* seg, int nseg) {
+    constexpr size_t CHUNK = 32u << 20;
+    constexpr int NTHREADS = 8;
+    static std::mutex mtx;
+    static char* stage[2] = {nullptr, nullptr};
+    std::lock_guard<std::mutex> lock(mtx);
+    std::vector<D2HSeg> todo;
+    for (int s = 0; s < nseg; s++) {
+        if (seg[s].bytes == 0) continue;
+        hipPointerAttribute_t at;
+        const bool pinned = hipPointerGetAttributes(&at, seg[s].dst) == hipSuccess &&
+                            (at.type == hipMemoryTypeHost || at.type == hipMemoryTypeDevice);
+        (void)hipGetLastError();  // pageable memory reports an error above
+        if (pinned) {
+            if (hipMemcpyAsync(seg[s].dst, seg[s].src, seg[s].bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return NLFEM_EDEVICE;
+        } else {
+            todo.push_back(seg[s]);
+        }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return NLFEM_EDEVICE;
+    if (todo.empty()) return NLFEM_OK;
+    if (!stage[0]) {
+        for (int b = 0; b < 2; b++)
+            if (hipHostMalloc((void**)&stage[b], CHUNK, hipHostMallocDefault) != hipSuccess) {
+                stage[b] = nullptr;
+                return NLFEM_EDEVICE;
+            }
+    }
+    // chunk list over the segments
+    struct Chunk { char* dst; const char* src; size_t bytes; };
+    std::vector<Chunk> ch;
+    for (const D2HSeg& s : todo)
+        for (size_t o = 0; o < s.bytes; o += CHUNK)
+            ch.push_back({(char*)s.dst + o, (const char*)s.src + o, std::min(CHUNK, s.bytes - o)});
+    hipEvent_t ev[2];
+    if (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess) return NLFEM_EDEVICE;
+    if (hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return NLFEM_EDEVICE;
+    }
+    int rc = NLFEM_OK;
+    auto issue = [&](size_t i) {
+        const int b = (int)(i & 1);
+        if (hipMemcpyAsync(stage[b], ch[i].src, ch[i].bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipEventRecord(ev[b], st) != hipSuccess)
+            rc = NLFEM_EDEVICE;
+    };
+    issue(0);
+    for (size_t i = 0; i < ch.size() && rc == NLFEM_OK; i++) {
+        // buffer (i + 1) & 1 was emptied by the host copy of chunk i - 1
+        if (i + 1 < ch.size()) issue(i + 1);
+        if (rc != NLFEM_OK || hipEventSynchronize(ev[i & 1]) != hipSuccess) {
+            rc = NLFEM_EDEVICE;
+            break;
+        }
+        const char* from = stage[i & 1];
+        const size_t n = ch[i].bytes, part = (n + NTHREADS - 1) / NTHREADS;
+        std::thread th[NTHREADS - 1];
+        for (int t = 1; t < NTHREADS; t++) {
+            const size_t o = std::min(n, t * part), e = std::min(n, o + part);
+            th[t - 1] = std::thread([=] { if (e > o) std::memcpy(ch[i].dst + o, from + o, e - o); });
+        }
+        std::memcpy(ch[i].dst, from, std::min(n, part));
+        for (auto& t : th) t.join();
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return rc;
+}
+
 extern "C" int nlfem_plan_csr_copy(const nlfem_plan* plan, long long* rowptr, int* col, double* val) {
     GUARD_BEGIN
     if (!plan || !plan->P.assembled) return fail(NLFEM_EINVAL, "plan not assembled");
     const Plan& P = plan->P;
-    if (hipMemcpy(rowptr, P.d_orow, sizeof(long long) * ((size_t)P.cfg.K + 1), hipMemcpyDeviceToHost) != hipSuccess ||
-        (P.nnz && hipMemcpy(col, P.d_ocol, sizeof(int) * (size_t)P.nnz, hipMemcpyDeviceToHost) != hipSuccess) ||
-        (P.nnz && hipMemcpy(val, P.d_oval, sizeof(double) * (size_t)P.nnz, hipMemcpyDeviceToHost) != hipSuccess))
-        return fail(NLFEM_EDEVICE, "device to host copy failed");
+    const D2HSeg seg[3] = {{rowptr, P.d_orow, sizeof(long long) * ((size_t)P.cfg.K + 1)},
+                           {col, P.d_ocol, sizeof(int) * (size_t)P.nnz},
+                           {val, P.d_oval, sizeof(double) * (size_t)P.nnz}};
+    if (copy_d2h_staged(P.stream, seg, 3) != NLFEM_OK) return fail(NLFEM_EDEVICE, "device to host copy failed");
     return NLFEM_OK;
     GUARD_END
 }

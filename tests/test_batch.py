@@ -9,6 +9,7 @@ rescaling perturbs coordinates by rounding, which tiny slivers of nearly tangent
 import numpy as np
 import pytest
 import scipy.sparse as sp
+from scipy.sparse.linalg import norm as spnorm
 
 import oracle
 from helpers import check_floor_usage, floor_usage
@@ -91,4 +92,44 @@ def test_batch_blocks_match_oracle():
               f"{n_floor} of {A.nnz} (worst floorless {worst_floorless:.3e})", flush=True)
         assert np.all(np.abs(D.data) <= bound)
         check_floor_usage(f"batch nE={m['nE']}", n_floor, worst_floorless, A.nnz)
+    assert st["pairs"] == total
+
+
+def _uneven():
+    """Eight meshes of uneven size and horizon, as in the batched workload: h from 1/8 to 1/32,
+    delta from below h (cut-dominated) to 6.4 h, vertex jitter 0.2 h."""
+    spec = [(8, 0.05), (8, 0.2), (16, 0.1), (16, 0.05), (24, 0.15), (32, 0.2), (32, 0.06), (12, 0.12)]
+    return [(unit_square(n, d, jitter=0.2, seed=20 + i, outdim=2), d) for i, (n, d) in enumerate(spec)]
+
+
+@pytest.mark.gpu
+def test_uneven_batch_blocks_match_oracle():
+    ms = _uneven()
+    conf = _conf()
+    b = batch.BatchAssembler([m for m, _ in ms], [d for _, d in ms], FIELD, conf)
+    try:
+        b.assemble()
+        blocks = b.matrices()
+        st = b.stats()
+    finally:
+        b.close()
+    total = 0
+    for (m, d), A in zip(ms, blocks):
+        k = dict(FIELD)
+        k["horizon"] = d
+        ref = oracle.par_system(m, k, conf)
+        total += ref["n_pairs"]
+        R = ref["A"].tocsr()
+        R.sort_indices()
+        assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+        Abs = oracle.abs_matrix(m, k, conf).tocsr()
+        D = (A - R).tocoo()
+        addends = np.asarray(Abs[D.row, D.col]).ravel()
+        floor = 1e-15 * abs(R).max()
+        n_floor, worst_floorless = floor_usage(D.data, addends, 1e-12, floor)
+        rel = float(spnorm(A - R) / spnorm(R))
+        print(f"[uneven batch] nE={m['nE']} delta={d} rel fro {rel:.2e}; entries needing the floor: {n_floor} "
+              f"of {A.nnz} (worst floorless {worst_floorless:.3e})", flush=True)
+        assert np.all(np.abs(D.data) <= 1e-12 * addends + floor)
+        check_floor_usage(f"uneven batch nE={m['nE']}", n_floor, worst_floorless, A.nnz)
     assert st["pairs"] == total
