@@ -292,6 +292,7 @@ struct BfsArgs {
     int2* closeList;
     long long closeCap;
     unsigned long long* counters;
+    const int* ord;   // processing order of the outer elements (tile_order), or null
 };
 
 // Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
@@ -691,8 +692,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     unsigned short* test = reinterpret_cast<unsigned short*>(hash + a.H);
     unsigned short* next = test + a.F;
     OuterPts& op = ops[hf];
-    const int aT = 2 * blockIdx.x + hf;
-    const bool live = aT < a.nE;
+    const int slot = 2 * blockIdx.x + hf;  // the two outer elements: consecutive ones of the tile order
+    const bool live = slot < a.nE;
+    const int aT = live && a.ord ? a.ord[slot] : slot;
     const int aTs = live ? aT : 0;
     const ElemRec A = a.elem[aTs];
     const bool skip = !live || A.label == 0;
@@ -928,7 +930,7 @@ template <bool FILL, bool PK = false>
 __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
                                                    const int* cntB, int* tcnt, const long long* tptr, int* tfill,
                                                    int* tadj, const long long* coff, int* tfillB, int* tcut,
-                                                   int capT) {
+                                                   int capT, const int* ord) {
     static_assert(!PK || FILL, "packed counts only in the fill pass");
     __shared__ int sh[PK ? 2 * TH : 3 * TH];
     int* hk = sh;
@@ -938,11 +940,14 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
     const int tid = threadIdx.x;
     for (int i = tid; i < TH; i += 256) { hk[i] = -1; hf[i] = 0; hb[i] = 0; }
     __syncthreads();
+    // the block's outer elements: tg consecutive ones of the tile order (ord), so that the
+    // blocks running at one time on one XCD write into the rows of a compact neighbourhood
     const int a0 = xcd_block(blockIdx.x, gridDim.x) * tg, a1 = min(a0 + tg, nE);
     // entries in groups of TU per thread: the group's list loads are issued before its LDS
     // updates, so the loads of one group overlap
     constexpr int TU = 4;
-    for (int aT = a0; aT < a1; aT++) {
+    for (int ia = a0; ia < a1; ia++) {
+        const int aT = ord ? ord[ia] : ia;
         const int nF = cntF[aT], n = nF + cntB[aT];
         for (int p0 = tid; p0 < n; p0 += TU * 256) {
             uint32_t ev[TU];
@@ -983,7 +988,8 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
     }
     if (!FILL) return;
     __syncthreads();
-    for (int aT = a0; aT < a1; aT++) {
+    for (int ia = a0; ia < a1; ia++) {
+        const int aT = ord ? ord[ia] : ia;
         const int nF = cntF[aT], n = nF + cntB[aT];
         const long long co = coff[aT];
         for (int p0 = tid; p0 < n; p0 += TU * 256) {
@@ -4098,6 +4104,50 @@ KernelParams kernel_params(const PlanConfig& c) {
 }
 
 // Upload the mesh and build the element records.
+// A processing order of the outer elements that groups them by square tiles of the bounding box
+// (about 16 x 16 element sizes each, row-major over the tiles), each tile's elements in index
+// order.  A kernel whose block b takes the elements ord[tg b .. tg b + tg) with the blocks of one
+// XCD on one contiguous range then has its concurrent blocks working in a compact patch of the
+// mesh instead of a strip of whole mesh rows, so the rows they write share the XCD's L2.
+static std::vector<int> tile_order(int nE, int nV, const long* tri, const double* v) {
+    std::vector<int> ord(nE);
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (int i = 0; i < nV; i++) {
+        x0 = std::min(x0, v[2 * i]); x1 = std::max(x1, v[2 * i]);
+        y0 = std::min(y0, v[2 * i + 1]); y1 = std::max(y1, v[2 * i + 1]);
+    }
+    const double area = (x1 - x0) * (y1 - y0);
+    double cell = 16.0 * std::sqrt(2.0 * area / std::max(nE, 1));
+    if (!(cell > 0) || !std::isfinite(cell)) {
+        for (int i = 0; i < nE; i++) ord[i] = i;
+        return ord;
+    }
+    long long ntx = 0, nty = 0;
+    for (;;) {
+        ntx = (long long)((x1 - x0) / cell) + 1;
+        nty = (long long)((y1 - y0) / cell) + 1;
+        if (ntx * nty <= (1LL << 22)) break;
+        cell *= 2.0;
+    }
+    std::vector<int> key(nE);
+    std::vector<int> start((size_t)(ntx * nty) + 1, 0);
+    for (int e = 0; e < nE; e++) {
+        double cx = 0, cy = 0;
+        for (int j = 0; j < 3; j++) {
+            const long vi = tri[3 * (size_t)e + j];
+            cx += v[2 * vi];
+            cy += v[2 * vi + 1];
+        }
+        const long long tx = std::min<long long>(ntx - 1, std::max<long long>(0, (long long)((cx / 3 - x0) / cell)));
+        const long long ty = std::min<long long>(nty - 1, std::max<long long>(0, (long long)((cy / 3 - y0) / cell)));
+        key[e] = (int)(ty * ntx + tx);
+        start[key[e] + 1]++;
+    }
+    for (size_t t = 1; t < start.size(); t++) start[t] += start[t - 1];
+    for (int e = 0; e < nE; e++) ord[start[key[e]]++] = e;
+    return ord;
+}
+
 void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* verts, const long* vlab64) {
     const PlanConfig& c = P.cfg;
     P.oo = c.outdim * c.outdim;
@@ -4125,6 +4175,9 @@ void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* ver
                                                    P.d_counters, c.nV);
     k_vlabel<<<nblk(c.nV, 256), 256, 0, P.stream>>>(c.nV, d_vlab64, P.d_vlabel);
     HIPCHECK(hipGetLastError());
+    const std::vector<int> ord = tile_order(c.nE, c.nV, tri64, verts);
+    dalloc(P.d_ord, (size_t)c.nE);
+    HIPCHECK(hipMemcpyAsync(P.d_ord, ord.data(), sizeof(int) * (size_t)c.nE, hipMemcpyHostToDevice, P.stream));
     if (c.nPg > 0) {
         std::vector<double> tg = duffy_tables(c.Pg, c.nPg, c.fractional_s);
         dalloc(P.d_tg, tg.size());
@@ -4299,6 +4352,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     ba.level0 = c.singular ? (c.method_close >= 2 ? 2 : c.method_close) : c.method_remote;
     ba.fullCC = c.fullCC;
     ba.cntF = nullptr;
+    ba.ord = P.d_ord;
     ba.closeCap = closeCap;
     // LDS sizing: the visited hash holds ~1.05x the elements of the delta+2h disc (load <= ~0.65
     // after rounding up to a power of two), the level buffers ~8 sqrt(disc) (a ring of the
@@ -4404,10 +4458,11 @@ void plan_assemble(Plan& P, const Estimates& est) {
         if (P.cap < 32768)
             k_transpose<true, true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
                                                              P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut,
-                                                             P.cap);
+                                                             P.cap, P.d_ord);
         else
             k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
-                                                       P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, P.cap);
+                                                       P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, P.cap,
+                                                       P.d_ord);
         k_tcheck<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tF, P.d_tcB, P.d_counters);
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
@@ -4422,7 +4477,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     if (!tdone) {
         HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
         k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
-                                                    nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr, 0, P.d_ord);
         scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
         long long ntadj = 0;
         HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -4432,7 +4487,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
         HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
         k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
-                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, 0);
+                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, 0, P.d_ord);
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);
@@ -4971,7 +5026,7 @@ void plan_free(Plan& P) {
     void* sp[] = {P.sur.d_wf, P.sur.d_bias, P.sur.d_norm, P.d_rlist, P.d_roff, P.d_rcnt};
     for (void* p : sp)
         if (p) (void)hipFree(p);
-    void* ptrs[] = {P.d_tri, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
+    void* ptrs[] = {P.d_tri, P.d_ord, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
                     P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF, P.d_zptr,

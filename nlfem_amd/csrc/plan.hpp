@@ -79,6 +79,7 @@ struct Plan {
 
     // mesh
     int* d_tri = nullptr;        // [nE*3] vertex ids
+    int* d_ord = nullptr;        // [nE] outer elements grouped by spatial tiles (tile_order)
     int* d_vlabel = nullptr;     // [nV]
     ElemRec* d_elem = nullptr;   // [nE]
     // vertex -> element incidence
