@@ -1435,6 +1435,8 @@ struct NumArgs {
     // domain-decomposition weights (null: 1): subdomain rows of the elements, sorted
     const long long* zptr;
     const int* zidx;
+    // mirror sums of k_cutw (NC summed components per node-pair entry), merged by k_mirror
+    double* mir;
 };
 
 // Weight of pair (a, b) under a domain decomposition: 1 / |Z_a n Z_b|, the number of subdomains
@@ -2948,6 +2950,312 @@ __global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? 3 : 1) void k_cu
             v += absA * a.q.dx[k] * a.q.psix[3 * k + i] * a.q.psix[3 * k + j] * tls[k * OO + OD * ao + bo];
         const long long p = a.posTT[9 * aT + 3 * i + j];
         atomicAdd(&a.val[p * OO + ao * OD + bo], v);
+    }
+}
+
+// Cut pairs with one item (pair x outer point) per lane, for the cases k_cut8 does not take
+// (outdim 2, DG, outer rules of 9..16 points): one 256-thread block per outer element, groups
+// of nPx lanes (one group per cut pair, one lane per outer point: 4 pairs per wavefront with
+// the 16-point rule), every wavefront walking its own pairs without block barriers.
+//   - A lane retriangulates B(x_k, delta) n bT into its LDS point list (PointCollector, one
+//     column of pts per lane), then integrates its fan triangles (R0, R_t+1, R_t+2) in one loop
+//     to the largest triangle count of the wavefront: the inner rule (kernel, pulled-back
+//     basis, the sums of inner_point) is issued once per triangle index, not once per place in
+//     the retriangulation where a point can be emitted.
+//   - The lane's shares of termNonloc (|aT| dx_k psi_i(x_k) innerNonloc_j) and termLocalPrime
+//     (dx_k |aT| sLP) are summed per pair with LDS FP64 atomics into the group's slot (no
+//     staging of per-item sums); innerLocal stays in the lane over all its pairs.
+//   - The group's lanes then scatter the pair's entries as cut_scatter does
+//     (src/Cassemble.cpp:721-770), and termLocal is formed once per block at the end.
+// KSYM (the microelastic field kernel, k01 = k10): components 00, 01, 11 are summed.
+template <int OD, int KSYM_>
+struct CutwLayout {
+    static constexpr int OO = OD * OD;
+    static constexpr int NC = KSYM_ ? 3 : OO;   // summed components per 2 x 2 block
+    static constexpr int PACC = 15 * NC;        // termNonloc 9 NC, termLocalPrime 6 NC
+};
+static size_t cutw_lds_bytes(int nPx, int od, bool ksym, int HN) {
+    const int nc = ksym ? 3 : od * od;
+    const int G = 64 / (nPx == 16 ? 16 : (nPx <= 8 ? 8 : std::max(nPx, 1)));
+    return 16 * (size_t)CUT_MAXP * 256 + 8 * (size_t)4 * G * 15 * nc + 10 * (size_t)HN;
+}
+static bool cutw_fits(int nPx) { return nPx >= 1 && nPx <= 16; }
+
+// Polygon points of one lane in column `tid` of pts[CUT_MAXP][256].
+struct PointColumn {
+    double2* col;  // &pts[tid]
+    int nt;
+    NLFEM_DEV void operator()(double r0, double r1, double s0, double s1, double t0, double t1) {
+        if (nt == 0) {
+            col[0] = make_double2(r0, r1);
+            col[256] = make_double2(s0, s1);
+            col[512] = make_double2(t0, t1);
+        } else if (nt + 2 < CUT_MAXP) {
+            col[(nt + 2) * 256] = make_double2(t0, t1);
+        }
+        nt++;
+    }
+};
+
+// Sum of NV values over the GSZ lanes of a group (GSZ = 8 or 16, groups aligned to 8 or 16
+// lanes), scattered: after log2(GSZ) xor steps, each of which keeps half of the remaining values
+// (the lower half where the step's bit of k is 0), lane k holds the sums of values
+// k NV / GSZ .. k NV / GSZ + NV / GSZ - 1 in out[].  share(e) gives this lane's value e.
+template <int GSZ, int NV, class F>
+NLFEM_DEV void group_reduce_scatter(F&& share, int k, double (&out)[NV / GSZ]) {
+    static_assert((GSZ == 8 || GSZ == 16) && NV % GSZ == 0, "groups of 8 or 16 lanes");
+    constexpr int H = NV / 2;
+    double r[H];
+    {
+        const bool up = (k & (GSZ / 2)) != 0;
+#pragma unroll
+        for (int e = 0; e < H; e++) {
+            const double lo = share(e), hi = share(H + e);
+            r[e] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, GSZ / 2, 64);
+        }
+    }
+#pragma unroll
+    for (int m = GSZ / 4, h = H / 2; m >= 1; m >>= 1, h >>= 1) {
+        const bool up = (k & m) != 0;
+#pragma unroll
+        for (int e = 0; e < h; e++) {
+            const double snd = up ? r[e] : r[h + e];
+            r[e] = (up ? r[h + e] : r[e]) + __shfl_xor(snd, m, 64);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NV / GSZ; u++) out[u] = r[u];
+}
+
+NLFEM_DEV void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// GSZ = 16 or 8: groups of GSZ lanes (an outer rule of 16 points, or of at most 8 with the lanes
+// beyond it idle), aligned to DPP rows, whose pair sums are reduced and scattered across the
+// group's lanes in registers (group_reduce_scatter); GSZ = 0: groups of nPx lanes whose pair
+// sums go through LDS atomics.
+template <int KER, int OD, typename T, int GSZ>
+__global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? (GSZ == 16 ? 2 : 3) : 1) void k_cutw(NumArgs a) {
+    constexpr int OO = OD * OD, N = 3 * OD, NB = 9 * OO;
+    constexpr bool KSYM = KER == KER_MICRO_FIELD;
+    using L = CutwLayout<OD, KSYM>;
+    constexpr int NC = L::NC, PACC = L::PACC;
+    // summed component c -> component of the OD x OD kernel block (KSYM: 00, 01, 11)
+    auto oc = [](int c) { return (KSYM && c == 2) ? 3 : c; };
+    // component o = ao OD + bo -> summed component
+    auto co = [](int o) { return KSYM ? (o == 3 ? 2 : (o == 2 ? 1 : o)) : o; };
+    extern __shared__ double ldsw[];
+    double2* pts = (double2*)ldsw;                            // [CUT_MAXP][256]
+    double* pacc = (double*)(pts + CUT_MAXP * 256);           // [4][G][PACC]
+    const int nPx = GSZ == 16 ? 16 : a.q.nPx;
+    const int GS = GSZ ? GSZ : nPx, G = 64 / GS;
+    int* hk = (int*)(pacc + 4 * G * PACC);                    // [HN]
+    unsigned short* hv = (unsigned short*)(hk + a.HN);        // [HN][3] positions in aT's rows
+    __shared__ double red[4][NB];
+    const int aT = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const ElemRec A = a.elem[aT];
+    if (A.label == 0) return;
+    const int nB = a.sur.list_in ? a.sur.rcnt[aT] : a.cntB[aT];
+    if (nB == 0) return;
+    const int gl = lane / GS, k = lane - gl * GS;
+    const bool lane_on = gl < G;
+    const bool kin = lane_on && k < nPx;  // the lane carries outer point k
+    int rA[3];
+    bool labA[3];
+    for (int j = 0; j < 3; j++) {
+        rA[j] = node_of(a.tri, aT, j, a.is_DG);
+        labA[j] = row_labelled(a.vlabel, rA[j], a.is_DG);
+    }
+    for (int i = tid; i < a.HN; i += 256) hk[i] = -1;
+    __syncthreads();
+    for (int j = 0; j < 3; j++) {
+        const long long r0 = a.rowptr[rA[j]];
+        const int len = (int)(a.rowptr[rA[j] + 1] - r0);
+        for (int i = tid; i < len; i += 256) {
+            int s = lds_slot(hk, a.HN, a.col[r0 + i]);
+            if (s >= 0) hv[3 * s + j] = (unsigned short)i;
+            else atomicOr(&a.counters[CNT_ERROR], 16ull);
+        }
+    }
+    __syncthreads();
+    // this lane's outer point x_k and weights |aT| dx_k psi_i(x_k), dx_k |aT|
+    double x[2] = {0, 0}, wa[3] = {0, 0, 0}, psx[3] = {0, 0, 0}, wd = 0;
+    if (kin) {
+        toPhys(A.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+        const double c1 = A.absDet * a.q.dx[k];
+        for (int i = 0; i < 3; i++) {
+            psx[i] = a.q.psix[3 * k + i];
+            wa[i] = c1 * psx[i];
+        }
+        wd = a.q.dx[k] * A.absDet;
+    }
+    double inLs[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) inLs[c] = 0;
+    unsigned mytri = 0;
+    double* pa = pacc + (w * G + (lane_on ? gl : 0)) * PACC;
+    double2* col = pts + tid;
+    int b = 0;
+    // Sum e of the current pair (layout: termNonloc (3 i + j) NC + c, then termLocalPrime
+    // 9 NC + q NC + c) into the mirror sums M (src/Cassemble.cpp:721-770).  The reference adds
+    // -termNonloc (i, j) to (aAdx[i], bAdx[j]) and to (bAdx[j], aAdx[i]), and termLocalPrime (i, j)
+    // to (bAdx[j], bAdx[i]) for all i, j; for a symmetric kernel both are symmetric, so M takes
+    // -termNonloc once at (aAdx[i], bAdx[j]) and termLocalPrime of i <= j once at (bAdx[j],
+    // bAdx[i]) (halved on the diagonal), and k_mirror adds M + M^T into the rows that are
+    // assembled: 15 NC atomics per pair instead of 27 OO.
+    auto scatter_sum = [&](int e, double v) {
+        if (e >= PACC || v == 0.0) return;
+        if (e < 9 * NC) {
+            const int ij = e / NC, c = e - ij * NC, i = ij / 3, j = ij - 3 * i;
+            const int sl = lds_find(hk, a.HN, node_of(a.tri, b, j, a.is_DG));
+            if (sl < 0) { atomicOr(&a.counters[CNT_ERROR], 8ull); return; }
+            const long long pos = a.rowptr[rA[i]] + hv[3 * sl + i];  // entry (aAdx[i], bAdx[j])
+            atomicAdd(&a.mir[pos * NC + c], -v);
+        } else {
+            const int q = (e - 9 * NC) / NC, c = (e - 9 * NC) - q * NC;
+            const int i0 = q < 3 ? 0 : (q < 5 ? 1 : 2);
+            const int j0 = q < 3 ? q : (q < 5 ? q - 2 : 2);
+            const long long qp = a.posTT[9 * b + 3 * j0 + i0];  // entry (bAdx[j0], bAdx[i0])
+            atomicAdd(&a.mir[qp * NC + c], i0 == j0 ? 0.5 * v : v);
+        }
+    };
+    for (int c0 = 0; c0 < nB; c0 += 4 * G) {
+        if (c0 + w * G >= nB) break;  // the whole wavefront has no pair left
+        const int p = c0 + w * G + gl;
+        const bool pon = lane_on && p < nB;
+        b = !pon ? 0
+                 : (a.sur.list_in ? a.sur.rlist[a.sur.roff[aT] + p]
+                                  : (int)(a.nbr[(size_t)aT * a.cap + a.cap - 1 - p] & IDX_MASK));
+        const ElemRec B = a.elem[b];
+        int nt = 0;
+        if (pon && kin && !tri_far(x[0], x[1], B.E, a.sq_far)) {
+            PointColumn pc{col, 0};
+            retriangulate<false>(x[0], x[1], B.E, a.kp.sqdelta, a.caps != 0, pc);
+            if (pc.nt > CUT_MAXP - 2) atomicOr(&a.counters[CNT_ERROR], 256ull);
+            nt = min(pc.nt, CUT_MAXP - 2);
+        }
+        mytri += nt;
+        int mt = nt;
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) mt = max(mt, __shfl_xor(mt, m, 64));
+        double inL[OO], inN[3 * OO], sLP[6 * OO];
+#pragma unroll
+        for (int o = 0; o < OO; o++) inL[o] = 0;
+#pragma unroll
+        for (int o = 0; o < 3 * OO; o++) inN[o] = 0;
+#pragma unroll
+        for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
+        if (mt > 0) {
+            const PivotLU lu = pivot_lu(B.E);
+            const double2 R0 = col[0];
+            for (int t = 0; t < mt; t++) {
+                if (t < nt) {
+                    const double2 R1 = col[(t + 1) * 256], R2 = col[(t + 2) * 256];
+                    const double rT[6] = {R0.x, R0.y, R1.x, R1.y, R2.x, R2.y};
+                    const double rTdet = absDet(rT);
+                    const FanBasis fb(lu, rT);
+                    for (int i = 0; i < a.q.nPy; i++) {
+                        double y[2];
+                        toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
+                        T K[OO];
+                        kernel_eval<KER, OO, T>((T)(x[0] - y[0]), (T)(x[1] - y[1]), a.kp, K);
+                        double p0, p1;
+                        fb.at(a.q.Py[2 * i], a.q.Py[2 * i + 1], p0, p1);
+                        T psi[3] = {(T)(1 - p0 - p1), (T)p0, (T)p1};
+                        inner_point<OD, T, true, true, true, KSYM>(K, (T)(a.q.dy[i] * rTdet), psi, inL, inN, sLP);
+                    }
+                }
+            }
+        }
+        if constexpr (GSZ != 0) {
+            // the 15 NC shares of the pair (padded to a multiple of GSZ), summed over the
+            // group's lanes by a reduce-scatter; lane k ends with sums k NV / GSZ .. and
+            // scatters them itself
+            constexpr int NV = (PACC + GSZ - 1) / GSZ * GSZ;
+            auto share = [&](int e) -> double {
+#pragma clang fp contract(fast)
+                if (!pon || nt == 0) return 0.0;
+                if (e < 9 * NC) {
+                    const int ij = e / NC, c = e - ij * NC;
+                    return wa[ij / 3] * inN[(ij % 3) * OO + oc(c)];
+                }
+                if (e < 15 * NC) {
+                    const int q = (e - 9 * NC) / NC, c = (e - 9 * NC) - q * NC;
+                    return wd * sLP[q * OO + oc(c)];
+                }
+                return 0.0;
+            };
+            double r[NV / GSZ];
+            group_reduce_scatter<GSZ, NV>(share, k, r);
+            if (pon && nt > 0) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) inLs[c] += inL[oc(c)];
+            }
+            if (pon) {
+#pragma unroll
+                for (int u = 0; u < NV / GSZ; u++) scatter_sum(k * (NV / GSZ) + u, r[u]);
+            }
+        } else {
+            // the group's pair sums: cleared, then every lane adds its shares
+            if (pon)
+                for (int e = k; e < PACC; e += GS) pa[e] = 0.0;
+            wave_sync_lds();
+            if (pon && nt > 0) {
+#pragma clang fp contract(fast)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+#pragma unroll
+                        for (int c = 0; c < NC; c++) atomicAdd(&pa[(3 * i + j) * NC + c], wa[i] * inN[j * OO + oc(c)]);
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) atomicAdd(&pa[9 * NC + q * NC + c], wd * sLP[q * OO + oc(c)]);
+#pragma unroll
+                for (int c = 0; c < NC; c++) inLs[c] += inL[oc(c)];
+            }
+            wave_sync_lds();
+            // the group's lanes scatter the pair's sums
+            if (pon)
+                for (int e = k; e < PACC; e += GS) scatter_sum(e, pa[e]);
+            wave_sync_lds();
+        }
+    }
+    unsigned tri = mytri;
+    for (int m = 32; m > 0; m >>= 1) tri += __shfl_xor(tri, m, 64);
+    if (lane == 0 && tri) atomicAdd(&a.counters[CNT_SUBTRI], (unsigned long long)tri);
+    // termLocal[N (i OD + ao) + j OD + bo] = sum_k |aT| dx_k psi_i psi_j (x_k) innerLocal_k[(ao, bo)]
+    double tL[NB];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int o = 0; o < OO; o++)
+                tL[N * (i * OD + o / OD) + j * OD + o % OD] = lane_on ? wa[i] * psx[j] * inLs[co(o)] : 0.0;
+    cut_flush_local<OD>(a, aT, tL, labA, red);
+}
+
+// Mirror sums of k_cutw into the matrix: entry (r, c), component (x, y) gets M(r, c)[(x, y)] +
+// M(c, r)[(y, x)] when row r is assembled (a labelled vertex, or DG).  One thread per entry, after
+// the cut kernel; KSYM: M holds the components 00, 01 (= 10), 11.
+template <int OD, bool KSYM>
+__global__ __launch_bounds__(256) void k_mirror(long long nnz, const int* rowOf, const int* tpos, const int* vlabel,
+                                                int is_DG, const double* mir, double* val) {
+    constexpr int OO = OD * OD, NC = KSYM ? 3 : OO;
+    const long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= nnz || !row_labelled(vlabel, rowOf[p], is_DG)) return;
+    const long long tp = tpos[p];
+    auto ci = [](int x, int y) { return KSYM ? x + y : x * OD + y; };
+#pragma unroll
+    for (int o = 0; o < OO; o++) {
+        const int x = o / OD, y = o % OD;
+        const double v = mir[p * NC + ci(x, y)] + mir[tp * NC + ci(y, x)];
+        if (v != 0.0) val[p * OO + o] += v;
     }
 }
 
@@ -4729,6 +5037,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
 
     ev_begin(P, PH_CUT);
     P.cut_items_used = 0;
+    P.cutw_used = false;
     bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
@@ -4736,9 +5045,14 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // chunk buffers and 10 B per slot); rows beyond it (very large horizons) take the per-pair
     // kernel, whose scatter searches the rows in global memory
     const int cut_HN = (int)next_pow2((long long)(1.3 * std::max(P.maxRow, 1)));
-    const bool use_cut8 = c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG;
+    // k_cutw (cut_items 0) where its LDS fits; the constant kernel keeps k_cut8, whose items
+    // are integrated from polygon moments
+    const bool cutw_ok = P.cut_items == 0 && cutw_fits(c.q.nPx) && P.maxRow <= 65535 &&
+                         cutw_lds_bytes(c.q.nPx, c.outdim, c.kernel == KER_MICRO_FIELD, cut_HN) <= 160 * 1024;
+    const bool use_cut8 = c.outdim == 1 && c.q.nPx <= 8 && !c.is_DG && (c.kernel == KER_CONSTANT || !cutw_ok);
     const bool cut_fits = use_cut8 ? (size_t)cut_HN * 28 <= 160 * 1024
-                                   : cut_lds_bytes(c.q.nPx, P.oo, cut_HN, CUT_ITEMS) <= 160 * 1024 && P.maxRow <= 65535;
+                                   : cutw_ok || (cut_lds_bytes(c.q.nPx, P.oo, cut_HN, CUT_ITEMS) <= 160 * 1024 &&
+                                                 P.maxRow <= 65535);
     if (P.d_zptr) {
         DISPATCH_PAIRS_Z(1);
     } else if (P.stats.bary > 0 || c.q.nPx > CUT_ITEMS || c.q.nPx * CUT_MAXPAIRS < CUT_ITEMS || !cut_fits) {
@@ -4840,6 +5154,47 @@ void plan_assemble(Plan& P, const Estimates& est) {
                 default: LAUNCH_CUT8(KER_MICRO); break;
             }
             gather = nc.stage != nullptr;
+        } else if (cutw_ok) {
+            // one item per lane, no chunk barriers (k_cutw)
+            const size_t lw = cutw_lds_bytes(c.q.nPx, c.outdim, c.kernel == KER_MICRO_FIELD, nc.HN);
+#define LAUNCH_CUTW_T(KER, OD, T)                                                                                   \
+    do {                                                                                                           \
+        if (c.q.nPx == 16) {                                                                                       \
+            if (lw > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cutw<KER, OD, T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw); \
+            k_cutw<KER, OD, T, 16><<<c.nE, 256, lw, st>>>(nc);                                                     \
+        } else if (c.q.nPx <= 8) {                                                                                 \
+            if (lw > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cutw<KER, OD, T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw); \
+            k_cutw<KER, OD, T, 8><<<c.nE, 256, lw, st>>>(nc);                                                      \
+        } else {                                                                                                   \
+            if (lw > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_cutw<KER, OD, T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw); \
+            k_cutw<KER, OD, T, 0><<<c.nE, 256, lw, st>>>(nc);                                                      \
+        }                                                                                                          \
+    } while (0)
+#define LAUNCH_CUTW(KER, OD)                                       \
+    do {                                                          \
+        if (c.precision == 1) LAUNCH_CUTW_T(KER, OD, float);      \
+        else LAUNCH_CUTW_T(KER, OD, double);                      \
+    } while (0)
+            P.cutw_used = true;
+            const int nmc = c.kernel == KER_MICRO_FIELD ? 3 : P.oo;
+            dreserve(P, P.d_mir, (size_t)P.nnz_nodes * nmc);
+            HIPCHECK(hipMemsetAsync(P.d_mir, 0, sizeof(double) * (size_t)P.nnz_nodes * nmc, st));
+            nc.mir = P.d_mir;
+            switch (c.kernel) {
+                case KER_CONSTANT: LAUNCH_CUTW(KER_CONSTANT, 1); break;
+                case KER_PARABOLA: LAUNCH_CUTW(KER_PARABOLA, 1); break;
+                case KER_FRACTIONAL: LAUNCH_CUTW(KER_FRACTIONAL, 1); break;
+                case KER_MICRO: LAUNCH_CUTW(KER_MICRO, 1); break;
+                default: LAUNCH_CUTW(KER_MICRO_FIELD, 2); break;
+            }
+            if (c.kernel == KER_MICRO_FIELD)
+                k_mirror<2, true><<<nblk(P.nnz_nodes, 256), 256, 0, st>>>(P.nnz_nodes, P.d_wrowOf, P.d_tpos, P.d_vlabel,
+                                                                          c.is_DG, P.d_mir, P.d_val);
+            else
+                k_mirror<1, false><<<nblk(P.nnz_nodes, 256), 256, 0, st>>>(P.nnz_nodes, P.d_wrowOf, P.d_tpos,
+                                                                           P.d_vlabel, c.is_DG, P.d_mir, P.d_val);
+#undef LAUNCH_CUTW
+#undef LAUNCH_CUTW_T
         } else {
             switch (c.kernel) {
                 case KER_CONSTANT: LAUNCH_CUT(KER_CONSTANT, 1); break;
@@ -5026,7 +5381,7 @@ void plan_free(Plan& P) {
     void* sp[] = {P.sur.d_wf, P.sur.d_bias, P.sur.d_norm, P.d_rlist, P.d_roff, P.d_rcnt};
     for (void* p : sp)
         if (p) (void)hipFree(p);
-    void* ptrs[] = {P.d_tri, P.d_ord, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
+    void* ptrs[] = {P.d_tri, P.d_ord, P.d_mir, P.d_vlabel, P.d_elem, P.d_vptr, P.d_vel, P.d_xadj, P.d_adj, P.d_nbr, P.d_cntF,
                     P.d_cntB, P.d_closeList, P.d_counters, P.d_tptr, P.d_tadj, P.d_rowptr, P.d_col, P.d_tpos,
                     P.d_posTT, P.d_val, P.d_tg, P.d_dg, P.d_orow, P.d_ocol, P.d_oval, P.d_scan_tmp,
                     P.d_poff, P.d_P, P.d_adjp, P.d_coff, P.d_tcB, P.d_tcut, P.d_stage, P.d_tF, P.d_zptr,

@@ -417,7 +417,8 @@ extern "C" int nlfem_plan_force_paths(nlfem_plan* plan, int flags) {
 
 extern "C" int nlfem_plan_set_cut_items(nlfem_plan* plan, int items) {
     if (!plan) return fail(NLFEM_EINVAL, "null plan");
-    if (items != 64 && items != 128 && items != 256) return fail(NLFEM_EINVAL, "items per chunk must be 64, 128 or 256");
+    if (items != 0 && items != 64 && items != 128 && items != 256)
+        return fail(NLFEM_EINVAL, "items per chunk must be 0 (one item per lane), 64, 128 or 256");
     plan->P.cut_items = items;
     return NLFEM_OK;
 }
@@ -426,17 +427,19 @@ extern "C" int nlfem_plan_autotune(nlfem_plan* plan, int* chosen, double* ms) {
     GUARD_BEGIN
     if (!plan) return fail(NLFEM_EINVAL, "null plan");
     Plan& P = plan->P;
-    static const int cand[3] = {64, 128, 256};
-    double t[3] = {0, 0, 0};
+    // candidates: chunks of 64, 128, 256 items for k_cut, and 0 = k_cutw (one item per lane)
+    static const int cand[4] = {64, 128, 256, 0};
+    double t[4] = {0, 0, 0, 0};
     int best = 1;
     P.cut_items = cand[best];
     plan_assemble(P, plan->est);
     if (P.cut_items_used == cand[best]) {
         t[best] = P.stats.ms[PH_CUT];
-        for (int i : {0, 2}) {
+        for (int i : {0, 2, 3}) {
             P.cut_items = cand[i];
             plan_assemble(P, plan->est);
-            if (P.cut_items_used != cand[i]) continue;  // the outer rule or the LDS does not allow it
+            const bool ran = cand[i] ? P.cut_items_used == cand[i] : P.cutw_used;
+            if (!ran) continue;  // the outer rule or the LDS does not allow it
             t[i] = P.stats.ms[PH_CUT];
             if (t[i] < t[best]) best = i;
         }
@@ -444,7 +447,7 @@ extern "C" int nlfem_plan_autotune(nlfem_plan* plan, int* chosen, double* ms) {
     P.cut_items = cand[best];
     if (chosen) *chosen = cand[best];
     if (ms)
-        for (int i = 0; i < 3; i++) ms[i] = t[i];
+        for (int i = 0; i < 4; i++) ms[i] = t[i];
     return NLFEM_OK;
     GUARD_END
 }

@@ -80,6 +80,7 @@ struct Plan {
     // mesh
     int* d_tri = nullptr;        // [nE*3] vertex ids
     int* d_ord = nullptr;        // [nE] outer elements grouped by spatial tiles (tile_order)
+    double* d_mir = nullptr;     // [nnz_nodes][NC] mirror sums of the per-lane cut kernel
     int* d_vlabel = nullptr;     // [nV]
     ElemRec* d_elem = nullptr;   // [nE]
     // vertex -> element incidence
@@ -144,8 +145,9 @@ struct Plan {
     std::unordered_map<const void*, size_t> dcap;
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
-    int cut_items = 128;      // items per chunk of the general cut kernel k_cut (64, 128, 256)
+    int cut_items = 0;        // cut kernel of the k_cut cases: 0 = k_cutw (one item per lane), or k_cut with 64, 128, 256 items per chunk
     int cut_items_used = 0;   // items per chunk k_cut ran with in the last assembly (0: k_cut not used)
+    bool cutw_used = false;   // the last assembly's cut pairs went through k_cutw (cut_items = 0)
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
     bool bfs_wave = false;  // one outer element per wavefront in the bitmap BFS (k_bfs instead of k_bfs_h)
     bool pat_nobm = false;  // the same for the pattern's element set

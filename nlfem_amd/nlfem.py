@@ -127,18 +127,20 @@ class Assembler:
         return self
 
     def set_cut_items(self, items):
-        """Items (pair x outer point) per chunk of the general cut kernel: 64, 128 or 256."""
+        """Items (pair x outer point) per chunk of the general cut kernel: 64, 128 or 256, or
+        0 for the per-lane kernel (one item per lane, no chunk barriers)."""
         check(lib().nlfem_plan_set_cut_items(self._h, int(items)))
         return self
 
     def autotune(self):
-        """Time the general cut kernel with 64, 128 and 256 items per chunk (one assembly each,
-        where the kernel runs and the size fits) and keep the fastest.  Returns the kept value
-        and the cut-phase milliseconds per candidate (0.0 where not tried)."""
+        """Time the general cut kernel with 64, 128 and 256 items per chunk and the per-lane
+        kernel (0), one assembly each where the kernel runs and fits, and keep the fastest.
+        Returns the kept value and the cut-phase milliseconds per candidate (0.0 where not
+        tried)."""
         chosen = C.c_int(0)
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         check(lib().nlfem_plan_autotune(self._h, C.byref(chosen), ms.ctypes.data_as(C.POINTER(C.c_double))))
-        return chosen.value, {64: float(ms[0]), 128: float(ms[1]), 256: float(ms[2])}
+        return chosen.value, {64: float(ms[0]), 128: float(ms[1]), 256: float(ms[2]), 0: float(ms[3])}
 
     def set_surrogate(self, model="default", tol=None, smin=None, smax=None, enabled=True):
         """Route cut pairs through the surrogate networks of `model` (a path written by

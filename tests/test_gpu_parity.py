@@ -310,8 +310,9 @@ def test_parity_general_paths():
 
 @pytest.mark.parametrize("name", ["C5s", "DG"])
 def test_cut_items_per_chunk_and_autotune(name):
-    """The general cut kernel with 64, 128 and 256 items per chunk, and after the autotune,
-    meets the FP64 bar against the oracle (the chunk size changes summation order only)."""
+    """The general cut kernel with 64, 128 and 256 items per chunk, the per-lane cut kernel
+    (items 0), and the plan after the autotune meet the FP64 bar against the oracle (the choice
+    changes summation order only)."""
     mesh, kernel, conf = case(name)
     ref = oracle.par_system(mesh, kernel, conf)
     Abs = oracle.abs_matrix(mesh, kernel, conf).tocsr()
@@ -328,7 +329,7 @@ def test_cut_items_per_chunk_and_autotune(name):
         assert same and np.all(np.abs(D.data) <= bound), label
         check_floor_usage(f"{name} {label}", n_floor, worst_floorless, A.nnz)
 
-    for items in (64, 128, 256):
+    for items in (64, 128, 256, 0):
         asm = nlfem.Assembler(mesh, kernel, conf).set_cut_items(items)
         try:
             check(asm.assemble().csr(), f"items={items}")
@@ -338,7 +339,7 @@ def test_cut_items_per_chunk_and_autotune(name):
     try:
         chosen, ms = asm.autotune()
         print(f"[parity] case={name} autotune chose {chosen}, cut ms {ms}", flush=True)
-        assert chosen in (64, 128, 256) and ms[128] > 0 and ms[chosen] > 0
+        assert chosen in (0, 64, 128, 256) and ms[128] > 0 and ms[0] > 0 and ms[chosen] > 0
         check(asm.csr(), "after autotune")
         check(asm.assemble().csr(), f"tuned items={chosen}")
     finally:
