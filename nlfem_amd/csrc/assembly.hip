@@ -2022,9 +2022,14 @@ NLFEM_DEV ColMap colmap_build(int* h, int H, const int* col, const long long* rp
 template <int KER, int OD, typename T, int NPY = 0>
 __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     constexpr int OO = OD * OD, NB = 9 * OO, N = 3 * OD;
+    // the field kernel with the unrolled inner rule sums k01 = k10 once, so every block it adds
+    // has equal components (0, 1) and (1, 0): the accumulators keep 00, 01, 11 (NCA = 3)
+    constexpr bool KS3 = NPY > 0 && KER == KER_MICRO_FIELD && OD == 2;
+    constexpr int NCA = KS3 ? 3 : OO;
+    auto aix = [](int ao, int bo) { return KS3 ? ao + bo : ao * OD + bo; };
     extern __shared__ double ldsd[];
-    double* acc = ldsd;                                  // [HN][3][OO]
-    int* hk = (int*)(acc + (size_t)a.HN * 3 * OO);       // [HN]
+    double* acc = ldsd;                                  // [HN][3][NCA]
+    int* hk = (int*)(acc + (size_t)a.HN * 3 * NCA);      // [HN]
     __shared__ double red[4][2 * NB];
     const int T0 = blockIdx.x;
     const int tid = threadIdx.x;
@@ -2038,7 +2043,7 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     }
     if (!(labA[0] || labA[1] || labA[2])) return;
     __shared__ int scan_tmp[4];
-    for (int i = tid; i < a.HN * 3 * OO; i += 256) acc[i] = 0.0;
+    for (int i = tid; i < a.HN * 3 * NCA; i += 256) acc[i] = 0.0;
     long long rp[3];
     int rlen[3];
     for (int j = 0; j < 3; j++) {
@@ -2130,7 +2135,8 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     for (int ao = 0; ao < OD; ao++)
 #pragma unroll
                         for (int bo = 0; bo < OD; bo++)
-                            atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], -tN[tni(i, ao, j, bo)]);
+                            if (!(KS3 && ao == 1 && bo == 0))
+                                atomicAdd(&acc[(s * 3 + i) * NCA + aix(ao, bo)], -tN[tni(i, ao, j, bo)]);
                 }
             }
         }
@@ -2208,7 +2214,8 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
                     for (int ao = 0; ao < OD; ao++)
 #pragma unroll
                         for (int bo = 0; bo < OD; bo++)
-                            atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], -tN[tni(i, ao, j, bo)]);
+                            if (!(KS3 && bo == 1 && ao == 0))
+                                atomicAdd(&acc[(s * 3 + j) * NCA + aix(bo, ao)], -tN[tni(i, ao, j, bo)]);
                 }
             }
         }
@@ -2366,11 +2373,11 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         if (t < NB) {  // +termLocal into (T_i, T_j)
             if (!labA[i]) continue;
             const int s = cm.find(rA[j]);
-            if (s >= 0) atomicAdd(&acc[(s * 3 + i) * OO + ao * OD + bo], v);
+            if (s >= 0 && !(KS3 && ao == 1 && bo == 0)) atomicAdd(&acc[(s * 3 + i) * NCA + aix(ao, bo)], v);
         } else {       // +termLocalPrime[N a + b] into (T_b, T_a): row T_j comp bo, col T_i comp ao
             if (!labA[j]) continue;
             const int s = cm.find(rA[i]);
-            if (s >= 0) atomicAdd(&acc[(s * 3 + j) * OO + bo * OD + ao], v);
+            if (s >= 0 && !(KS3 && bo == 1 && ao == 0)) atomicAdd(&acc[(s * 3 + j) * NCA + aix(bo, ao)], v);
         }
     }
     __syncthreads();
@@ -2383,7 +2390,8 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         for (int i = tid; i < len; i += 256) {
             const int s = cm.find(a.col[r0 + i]);
 #pragma unroll
-            for (int o = 0; o < OO; o++) dst[(long long)i * OO + o] = (s >= 0) ? acc[(s * 3 + j) * OO + o] : 0.0;
+            for (int o = 0; o < OO; o++)
+                dst[(long long)i * OO + o] = (s >= 0) ? acc[(s * 3 + j) * NCA + aix(o / OD, o % OD)] : 0.0;
         }
     }
 }
@@ -4966,7 +4974,9 @@ void plan_assemble(Plan& P, const Estimates& est) {
         for (int attempt = 0;; attempt++) {
             NumArgs ni = na;
             ni.HN = (int)hn;
-            const size_t lb = (size_t)hn * (24 * P.oo + 4);
+            // the field kernel with the 7-point inner rule accumulates 3 components (k01 = k10)
+            const int nca = (c.kernel == KER_MICRO_FIELD && c.q.nPy == 7) ? 3 : P.oo;
+            const size_t lb = (size_t)hn * (24 * nca + 4);
             if (lb > 160 * 1024) throw std::length_error("interior LDS table exceeds 160 KiB");
 #define LAUNCH_INT_N(KER, OD, NPY)                                                                                  \
     do {                                                                                                            \
