@@ -49,12 +49,14 @@ def algorithmic_flops(st, nPx, nPy):
 
 
 def cut_kernel_name(conf, kernel):
-    """The library's cut-pair kernel: k_cut8 for scalar kernels with <= 8 outer points and CG
-    (its 128-register instance k_cut8_o4 for the constant and parabola kernels), else k_cut."""
-    if not (kernel.get("outputdim", 1) == 1 and conf["quadrature"]["outer"]["points"].shape[0] <= 8
+    """The library's cut-pair kernel with the default plan: k_cut8_o4 for the constant kernel
+    with <= 8 outer points and CG (its items integrated from polygon moments), k_cutw (one item
+    per lane) for the other kernels and for outdim 2 or DG with <= 16 outer points, else k_cut."""
+    nPx = conf["quadrature"]["outer"]["points"].shape[0]
+    if (kernel["function"] == "constant" and kernel.get("outputdim", 1) == 1 and nPx <= 8
             and conf.get("ansatz", "CG") == "CG"):
-        return "k_cut"
-    return "k_cut8_o4" if kernel["function"] in ("constant", "parabola") else "k_cut8"
+        return "k_cut8_o4"
+    return "k_cutw" if nPx <= 16 else "k_cut"
 
 
 def measured_traffic(kernel_name, workload):

@@ -85,7 +85,10 @@ int nlfem_plan_create(nlfem_plan** out, int K, const long* ptrTriangles, const l
 int nlfem_plan_assemble(nlfem_plan* plan);
 /* K (matrix order) and nnz of the assembled CSR of A (rows sorted, exact zeros dropped). */
 int nlfem_plan_csr_info(const nlfem_plan* plan, long long* K, long long* nnz);
-/* Copies rowptr[K+1], col[nnz], val[nnz] to host memory. */
+/* Copies rowptr[K+1], col[nnz], val[nnz] to host memory.  Pageable destinations are filled
+ * through two pinned staging buffers (32 MB each, allocated on first use and kept for the
+ * process), with the DMA of one chunk overlapping the host copy of the previous one; pinned
+ * destinations are copied directly.  Calls are serialised by a lock. */
 int nlfem_plan_csr_copy(const nlfem_plan* plan, long long* rowptr, int* col, double* val);
 /* Device pointers of the CSR (valid until the next assemble or destroy). */
 int nlfem_plan_csr_device(const nlfem_plan* plan, long long** rowptr, int** col, double** val);
