@@ -105,6 +105,7 @@ def cpu_baseline(mesh, kernel, conf, budget_s=15.0, budget_1t_s=8.0):
     cores = os.cpu_count() or 1
     r, stride, n_outer = _cpu_sample(oracle, mesh, kernel, conf, budget_s, 0)
     out = {"value": r["n_pairs"] / r["seconds"], "unit": "element-pairs/s", "cores": cores, "kind": "port",
+           "cpu_model": _cpu_model(),
            "sample": f"every {stride}-th of {nE} outer elements ({n_outer}), {r['n_pairs']} pairs, "
                      f"{r['seconds']:.2f} s incl. dual graph and merge, OMP threads={cores}"}
     r1, s1, n1 = _cpu_sample(oracle, mesh, kernel, conf, budget_1t_s, 1)
@@ -112,6 +113,18 @@ def cpu_baseline(mesh, kernel, conf, budget_s=15.0, budget_1t_s=8.0):
                          "sample": f"every {s1}-th of {nE} outer elements ({n1}), {r1['n_pairs']} pairs, "
                                    f"{r1['seconds']:.2f} s"}
     return out
+
+
+def _cpu_model():
+    """The host CPU model (/proc/cpuinfo), reported with the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def end_to_end(mesh, kernel, conf, reps=3):

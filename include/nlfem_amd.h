@@ -129,17 +129,18 @@ int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double 
 #define NLFEM_PATH_TRANSPOSE_CSR 4
 #define NLFEM_PATH_BFS_WAVE 8
 int nlfem_plan_force_paths(nlfem_plan* plan, int flags);
-/* Cut kernel of the cases the 8-lane kernel does not take (outdim 2, DG, outer rules of more
- * than 8 points): items (pair x outer point) per chunk of the chunked kernel, 64, 128 (default)
- * or 256, i.e. 4, 8 or 16 pairs per 256-thread block with the 16-point rule (larger chunks keep
- * more threads busy per chunk and need more LDS per block); or 0: the per-lane kernel (one item
- * per lane, each wavefront on its own pairs, no chunk barriers; outer rules of up to 16
- * points).  No counterpart in the reference.  A value that the outer rule or the LDS does not
+/* Cut kernel for every kernel but the constant one and for outdim 2 and DG: 0 (default) the
+ * per-lane kernel (one item per lane, each wavefront on its own pairs, no chunk barriers;
+ * outer rules of up to 16 points); otherwise the chunked kernel with 64, 128 or 256 items
+ * (pair x outer point) per chunk, i.e. 4, 8 or 16 pairs per 256-thread block with the 16-point
+ * rule (larger chunks keep more threads busy per chunk and need more LDS per block), except
+ * that scalar CG kernels with at most 8 outer points take the 8-lane kernel for any nonzero
+ * value.  No counterpart in the reference.  A value that the outer rule or the LDS does not
  * allow falls back to 128 at launch. */
 int nlfem_plan_set_cut_items(nlfem_plan* plan, int items);
 /* Autotune of the cut kernel: assembles once with 128 items per chunk and, when the chunked
- * kernel ran, once with each of 64, 256 and 0 (the per-lane kernel) that fits; keeps the value
- * with the shortest cut phase.  *chosen (may be NULL) gets it, ms[4] (may be NULL) the
+ * kernel ran, once with each of 64, 256 and 0 (the per-lane kernel) that fits (when the 8-lane
+ * kernel ran instead, once with 0 only); keeps the value with the shortest cut phase.  *chosen (may be NULL) gets it, ms[4] (may be NULL) the
  * cut-phase times of 64, 128, 256 and 0 (0 where not tried).  The plan holds a complete
  * assembly afterwards (values do not depend on the choice beyond summation order).  No
  * counterpart in the reference. */

@@ -44,7 +44,9 @@ namespace nlfem {
 
 constexpr int CNT_PAIRS = 0, CNT_PROBES = 1, CNT_CLOSE = 2, CNT_INTERIOR = 3, CNT_CUT = 4, CNT_BARY = 5,
               CNT_OVERFLOW = 6, CNT_ERROR = 7, CNT_MAXLIST = 8, CNT_MAXROW = 9, CNT_MAXT = 10, CNT_SUBTRI = 11,
-              CNT_SUR_ACC = 12, CNT_SUR_REJ = 13, CNT_SUR_ITEMS = 14, CNT_SAMPLES = 15;
+              CNT_SUR_ACC = 12, CNT_SUR_REJ = 13, CNT_SUR_ITEMS = 14, CNT_SAMPLES = 15, CNT_MAXOFF = 16,
+              CNT_LABEL0 = 17;
+constexpr int NCNT = 24;  // counters allocated (CNT_* < NCNT)
 
 // ======================================================================= small utilities
 __global__ void k_fill_int(int* p, int v, long long n) {
@@ -882,6 +884,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 }
 
 NLFEM_DEV int list_index(int p, int nF, int cap) { return p < nF ? p : cap - 1 - (p - nF); }
+
+// Largest |b - aT| over the neighbour lists (CNT_MAXOFF) and whether the mesh has label-0
+// elements (CNT_LABEL0): one wavefront per outer element.  Without label-0 elements every
+// element a search visits is a listed element or a dual-graph neighbour of one, so this extent
+// plus the dual graph's index bandwidth bounds the search's index window.
+__global__ __launch_bounds__(256) void k_list_extent(int nE, int cap, const uint32_t* nbr, const int* cntF,
+                                                     const int* cntB, const ElemRec* elem,
+                                                     unsigned long long* counters) {
+    const int aT = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (aT >= nE) return;
+    if (lane == 0 && elem[aT].label == 0) atomicOr(&counters[CNT_LABEL0], 1ull);
+    const int nF = cntF[aT], n = nF + cntB[aT];
+    int m = 0;
+    for (int p = lane; p < n; p += 64) {
+        const int b = (int)(nbr[(size_t)aT * cap + list_index(p, nF, cap)] & IDX_MASK);
+        m = max(m, abs(b - aT));
+    }
+    for (int k = 32; k > 0; k >>= 1) m = max(m, __shfl_xor(m, k, 64));
+    if (lane == 0 && m > 0) atomicMax(&counters[CNT_MAXOFF], (unsigned long long)m);
+}
 
 // ======================================================================= transpose
 // Transposed relation.  Row b lists the outer elements aT of the pairs (aT, b): pairs from
@@ -4336,7 +4358,7 @@ static void dreserve(Plan& P, Tp*& p, size_t n) {
 }
 
 static void read_counters(Plan& P, unsigned long long* h) {
-    HIPCHECK(hipMemcpyAsync(h, P.d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, P.stream));
+    HIPCHECK(hipMemcpyAsync(h, P.d_counters, NCNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, P.stream));
     HIPCHECK(hipStreamSynchronize(P.stream));
 }
 
@@ -4485,8 +4507,8 @@ void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* ver
     dalloc(P.d_tri, 3 * (size_t)c.nE);
     dalloc(P.d_elem, (size_t)c.nE);
     dalloc(P.d_vlabel, (size_t)c.nV);
-    dalloc(P.d_counters, 16);
-    HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), P.stream));
+    dalloc(P.d_counters, NCNT);
+    HIPCHECK(hipMemsetAsync(P.d_counters, 0, NCNT * sizeof(unsigned long long), P.stream));
     k_setup<<<nblk(c.nE, 256), 256, 0, P.stream>>>(c.nE, d_tri64, d_lab64, d_verts, P.d_tri, P.d_elem,
                                                    P.d_counters, c.nV);
     k_vlabel<<<nblk(c.nV, 256), 256, 0, P.stream>>>(c.nV, d_vlab64, P.d_vlabel);
@@ -4505,7 +4527,7 @@ void plan_setup(Plan& P, const long* tri64, const long* lab64, const double* ver
     HIPCHECK(hipMalloc(&P.d_scan_tmp, scan_n * sizeof(long long)));
     P.scan_tmp_bytes = scan_n * sizeof(long long);
     HIPCHECK(hipStreamSynchronize(P.stream));
-    unsigned long long h[16];
+    unsigned long long h[NCNT];
     read_counters(P, h);
     (void)hipFree(d_tri64);
     (void)hipFree(d_lab64);
@@ -4608,8 +4630,8 @@ static void set_surrogate_args(const Plan& P, SurArgs& s) {
 void plan_assemble(Plan& P, const Estimates& est) {
     const PlanConfig& c = P.cfg;
     const hipStream_t st = P.stream;
-    unsigned long long h[16];
-    HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+    unsigned long long h[NCNT];
+    HIPCHECK(hipMemsetAsync(P.d_counters, 0, NCNT * sizeof(unsigned long long), st));
 
     // ---------------------------------------------------------------- dual graph
     ev_begin(P, PH_DUAL);
@@ -4683,7 +4705,21 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // the window is reported and the BFS reruns with the hash.
     const double hmin = 2.0 * est.amin / est.hmax;
     const long long levels = (long long)std::ceil((delta + 2.0 * est.hmax) / hmin) + 3;
-    const long long whalf = bandwidth * levels + 1;
+    // After a first BFS the plan knows the largest index offset any search visited
+    // (CNT_MAXOFF): later assemblies of the same mesh visit the same elements, so that extent
+    // is an exact window.  The level bound above takes the smallest element height of the
+    // mesh, far below the typical one on jittered or mixed meshes.  The measured window is
+    // taken where the estimated one keeps two searches from sharing a wavefront (16-bit
+    // offsets) or lets the LDS hold fewer than the 12 wavefronts per CU the registers allow;
+    // otherwise the estimate stays (a narrower window measured 2 % slower on the regular C4
+    // mesh, C2's jittered mesh 23 % faster).
+    long long whalf = bandwidth * levels + 1;
+    if (P.bfs_maxoff > 0) {
+        const long long hw = (2 * whalf + 31) / 32;
+        const long long fw = std::max<long long>(next_pow2(std::max<long long>(fvis, 256)), 256);
+        if (32 * hw > 65536 || 12 * 2 * (long long)sizeof(int) * (hw + fw) > 160 * 1024)
+            whalf = std::min(whalf, P.bfs_maxoff + 1);
+    }
     const long long bm_words = (2 * whalf + 31) / 32;
     bool use_bm = !c.fullCC && W4 > 0 && bm_words <= 8192 && !P.bfs_nobm;
     for (int attempt = 0;; attempt++) {
@@ -4708,7 +4744,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
         ba.cap = (int)cap; ba.H = (int)H; ba.F = (int)F;
         ba.nbr = P.d_nbr; ba.cntF = P.d_cntF; ba.cntB = P.d_cntB; ba.closeList = P.d_closeList;
         ba.counters = P.d_counters;
-        HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+        HIPCHECK(hipMemsetAsync(P.d_counters, 0, NCNT * sizeof(unsigned long long), st));
         ev_begin(P, PH_BFS);
         if (half) {
             if (ldsb > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_bfs_h, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
@@ -4737,6 +4773,16 @@ void plan_assemble(Plan& P, const Estimates& est) {
         visit = std::min<long long>(2 * visit, c.nE);
         hvis = std::min<long long>(2 * hvis, 2 * (long long)c.nE);
         fvis *= 2;
+    }
+    if (P.bfs_maxoff == 0 && !c.fullCC) {
+        // the index extent of the searches, measured once per plan from the lists (-1: the mesh
+        // has label-0 elements, which searches traverse without listing them)
+        k_list_extent<<<nblk(c.nE, 4), 256, 0, st>>>(c.nE, P.cap, P.d_nbr, P.d_cntF, P.d_cntB, P.d_elem,
+                                                     P.d_counters);
+        HIPCHECK(hipGetLastError());
+        unsigned long long hx[NCNT];
+        read_counters(P, hx);
+        P.bfs_maxoff = hx[CNT_LABEL0] ? -1 : (long long)hx[CNT_MAXOFF] + bandwidth;
     }
     P.stats.pairs = h[CNT_PAIRS];
     P.stats.probes = h[CNT_PROBES];
@@ -4823,7 +4869,10 @@ void plan_assemble(Plan& P, const Estimates& est) {
     long long rc = next_pow2((long long)(0.6 * (c.is_DG ? 4 * ecap : ecap)) + 64);
     // element set as a bitmap: the row's elements lie within levels + 2 dual-graph steps of
     // the row's first element, each step moving at most `bandwidth` indices
-    const long long pwhalf = bandwidth * (levels + 2) + 1;
+    // (with a measured list extent m: the row's elements are the list entries of the elements T
+    // that share the row's vertex, |T - first| <= bandwidth, so they lie within m + bandwidth)
+    const long long pwhalf = P.bfs_maxoff > 0 ? std::min(bandwidth * (levels + 2), P.bfs_maxoff) + 1
+                                              : bandwidth * (levels + 2) + 1;
     const long long pwords = next_pow2((2 * pwhalf + 31) / 32 + 8);  // 32*pwords is a multiple of 256
     bool pat_bm = !c.fullCC && !P.pat_nobm && pwords <= 4096;
     pa.whalf = (int)pwhalf;
@@ -4845,7 +4894,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
             if (pat_bm) { pat_bm = false; P.pat_nobm = true; continue; }
             glob = true;
         }
-        HIPCHECK(hipMemsetAsync(P.d_counters, 0, 16 * sizeof(unsigned long long), st));
+        HIPCHECK(hipMemsetAsync(P.d_counters, 0, NCNT * sizeof(unsigned long long), st));
         if (glob) {
             const int G = (int)std::min<long long>(P.nNodes, 1024);
             dreserve(P, P.d_gtab, (size_t)G * (size_t)(pa.EH + pa.DH));
@@ -5048,6 +5097,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     ev_begin(P, PH_CUT);
     P.cut_items_used = 0;
     P.cutw_used = false;
+    P.cut8_used = false;
     bool gather = false;  // termLocalPrime of the cut pairs staged for k_lp_gather
     P.stats.sur_acc = P.stats.sur_rej = P.stats.sur_items = 0;
     const bool use_sur = surrogate_applies(P);
@@ -5143,6 +5193,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     } while (0)
 #define LAUNCH_CUT8(KER) LAUNCH_CUT8_K(k_cut8, KER)
         if (use_cut8) {
+            P.cut8_used = true;
             long long ncut = 0;
             HIPCHECK(hipMemcpyAsync(&ncut, P.d_coff + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
@@ -5328,7 +5379,7 @@ long long plan_surrogate_samples(Plan& P, long long maxn, unsigned stride, unsig
     nc.sur.ntarg = NT;
     const size_t lb = cut_lds_bytes(c.q.nPx, P.oo, nc.HN);
     if (lb > 160 * 1024 || P.maxRow > 65535) throw std::length_error("cut-pair LDS exceeds 160 KiB");
-    unsigned long long h[16];
+    unsigned long long h[NCNT];
     try {
         HIPCHECK(hipMemsetAsync(P.d_counters + CNT_SAMPLES, 0, sizeof(unsigned long long), st));
         if (field) {

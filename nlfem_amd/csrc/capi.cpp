@@ -433,9 +433,13 @@ extern "C" int nlfem_plan_autotune(nlfem_plan* plan, int* chosen, double* ms) {
     int best = 1;
     P.cut_items = cand[best];
     plan_assemble(P, plan->est);
-    if (P.cut_items_used == cand[best]) {
+    // scalar kernels with at most 8 outer points take the 8-lane kernel k_cut8 for any
+    // nonzero value: it is timed once, as 128, against k_cutw
+    const bool cut8 = P.cut8_used;
+    if (P.cut_items_used == cand[best] || cut8) {
         t[best] = P.stats.ms[PH_CUT];
         for (int i : {0, 2, 3}) {
+            if (cut8 && i != 3) continue;
             P.cut_items = cand[i];
             plan_assemble(P, plan->est);
             const bool ran = cand[i] ? P.cut_items_used == cand[i] : P.cutw_used;

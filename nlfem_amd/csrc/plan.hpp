@@ -145,9 +145,11 @@ struct Plan {
     std::unordered_map<const void*, size_t> dcap;
 
     long long bfs_visit = 0, bfs_hvis = 0, bfs_fvis = 0;  // BFS capacities that fit (reused)
+    long long bfs_maxoff = 0;  // bound of |visited element - outer element| from the first BFS (0: not yet measured, -1: none)
     int cut_items = 0;        // cut kernel of the k_cut cases: 0 = k_cutw (one item per lane), or k_cut with 64, 128, 256 items per chunk
     int cut_items_used = 0;   // items per chunk k_cut ran with in the last assembly (0: k_cut not used)
     bool cutw_used = false;   // the last assembly's cut pairs went through k_cutw (cut_items = 0)
+    bool cut8_used = false;   // the last assembly's cut pairs went through k_cut8
     bool bfs_nobm = false;  // the bitmap visited set overflowed its window once: use the hash
     bool bfs_wave = false;  // one outer element per wavefront in the bitmap BFS (k_bfs instead of k_bfs_h)
     bool pat_nobm = false;  // the same for the pattern's element set

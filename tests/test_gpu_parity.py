@@ -308,7 +308,7 @@ def test_parity_general_paths():
                 paths=nlfem.Assembler.PATH_BFS_WAVE)
 
 
-@pytest.mark.parametrize("name", ["C5s", "DG"])
+@pytest.mark.parametrize("name", ["C5s", "DG", "C2s"])
 def test_cut_items_per_chunk_and_autotune(name):
     """The general cut kernel with 64, 128 and 256 items per chunk, the per-lane cut kernel
     (items 0), and the plan after the autotune meet the FP64 bar against the oracle (the choice
