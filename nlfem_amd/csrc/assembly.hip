@@ -26,6 +26,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nlfem_device.hpp"
@@ -4186,7 +4187,47 @@ __global__ __launch_bounds__(64) void k_close(CloseArgs a) {
     double acc[NACC];
 #pragma unroll
     for (int i = 0; i < NACC; i++) acc[i] = 0;
-    for (int it = lane; it < nT * a.nPg; it += 64) {
+    // integrate_fractional: the products psixy_i psixy_j are symmetric in (i, j) and vanish
+    // beyond the pair's 6 - nEqual local basis functions, so only the lower triangle of the
+    // first NL rows is summed (NL = 3, 4, 5 for identical, edge-sharing, vertex-sharing pairs)
+    // and mirrored after the wave reduction
+    auto frac_points = [&](auto nlc) {
+        constexpr int NL = decltype(nlc)::value;
+        for (int it = lane; it < nT * a.nPg; it += 64) {
+            const int k = it % a.nPg;
+            const double* t = a.tg + base + (long long)it * 10;
+            double xc[2], yc[2];
+            toPhys(as, t[4], t[5], xc);
+            toPhys(bs, t[6], t[7], yc);
+            double K[OO];
+            kernel_eval<KER, OO, double>(xc[0] - yc[0], xc[1] - yc[1], a.kp, K);
+            const double wdg = t[8] * a.dg[k];
+            const double px[3] = {1 - t[4] - t[5], t[4], t[5]};
+            const double py[3] = {1 - t[6] - t[7], t[6], t[7]};
+            double pxy[5];
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                const int ie = i < nEqual ? i : (i < 3 ? i : i - shift);
+                pxy[i] = i < nEqual ? px[i] - py[i] : (i < 3 ? px[i] : -py[ie]);
+            }
+#pragma unroll
+            for (int o = 0; o < OO; o++) {
+                const double f = K[o] * wdg;
+#pragma unroll
+                for (int i = 0; i < NL; i++) {
+                    const double fi = f * pxy[i];
+#pragma unroll
+                    for (int j = 0; j <= i; j++) acc[(i * 5 + j) * OO + o] += fi * pxy[j];
+                }
+            }
+        }
+    };
+    if constexpr (!WEAK) {
+        if (nLocal == 3) frac_points(std::integral_constant<int, 3>{});
+        else if (nLocal == 4) frac_points(std::integral_constant<int, 4>{});
+        else frac_points(std::integral_constant<int, 5>{});
+    }
+    for (int it = lane; WEAK && it < nT * a.nPg; it += 64) {
         const int k = it % a.nPg;
         const double* t = a.tg + base + (long long)it * 10;
         const double det = WEAK ? t[9] : t[8];
@@ -4210,26 +4251,18 @@ __global__ __launch_bounds__(64) void k_close(CloseArgs a) {
                         acc[NB + (i * 3 + j) * OO + o] += f * py[i] * py[j];
                         acc[2 * NB + (i * 3 + j) * OO + o] += f * px[i] * py[j];
                     }
-        } else {
-            double px[3] = {1 - t[4] - t[5], t[4], t[5]};
-            double py[3] = {1 - t[6] - t[7], t[6], t[7]};
-            double pxy[6] = {0, 0, 0, 0, 0, 0};
-            for (int i = 0; i < nEqual; i++) pxy[i] = px[i] - py[i];
-            for (int i = nEqual; i < 3; i++) { pxy[i] = px[i]; pxy[i + shift] = -py[i]; }
-#pragma unroll
-            for (int i = 0; i < 5; i++)
-#pragma unroll
-                for (int j = 0; j < 5; j++)
-#pragma unroll
-                    for (int o = 0; o < OO; o++) acc[(i * 5 + j) * OO + o] += K[o] * wdg * pxy[i] * pxy[j];
         }
     }
-    // wave reduction into LDS
+    // wave reduction into LDS (fractional: the lower triangle, mirrored)
 #pragma unroll
     for (int i = 0; i < NACC; i++) {
+        if (!WEAK && (i / OO) % 5 > (i / OO) / 5) continue;  // j > i
         double v = acc[i];
         for (int m = 32; m > 0; m >>= 1) v += shfl_xor_d(v, m);
-        if (lane == 0) accs[i] = v;
+        if (lane == 0) {
+            accs[i] = v;
+            if (!WEAK) accs[((i / OO) % 5 * 5 + (i / OO) / 5) * OO + i % OO] = v;
+        }
     }
     for (int i = lane; i < 4 * NB; i += 64) (&blk[0][0])[i] = 0;
     __syncthreads();

@@ -17,6 +17,8 @@ import oracle
 from helpers import (case, check_floor_usage, compare, fan_mesh, floor_usage, gmsh_case, label0_mesh, rigid_residuals,
                      sorted_lists)
 from nlfem_amd import nlfem
+from nlfem_amd.config import make_configuration
+from nlfem_amd.mesh import unit_square
 
 pytestmark = pytest.mark.gpu
 
@@ -368,3 +370,33 @@ def test_parity_odd_element_count():
     kern = {"function": "constant", "horizon": 0.2, "outputdim": 1}
     _check_case("odd", odd, kern, make_configuration())
     _check_case("odd-wave", odd, kern, make_configuration(), paths=nlfem.Assembler.PATH_BFS_WAVE)
+
+
+def _rule12():
+    """A 12-point outer rule: the 3-point degree-2 rule on each of the four midpoint
+    subtriangles of the reference triangle (weights 1/24, sum 1/2; chk_BasisFunction holds)."""
+    base = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]])
+    subs = [np.array([[0, 0], [0.5, 0], [0, 0.5]]), np.array([[0.5, 0], [1, 0], [0.5, 0.5]]),
+            np.array([[0, 0.5], [0.5, 0.5], [0, 1]]), np.array([[0.5, 0.5], [0, 0.5], [0.5, 0]])]
+    pts = []
+    for T in subs:
+        for p in base:
+            pts.append(T[0] + p[0] * (T[1] - T[0]) + p[1] * (T[2] - T[0]))
+    return np.array(pts), np.full(12, 1 / 24)
+
+
+@pytest.mark.parametrize("kern", ["parabola", "linearPrototypeMicroelasticField"])
+def test_parity_outer_rule_12_points(kern):
+    """Outer rules of 9..15 points take the per-lane cut kernel with groups of nPx lanes whose
+    pair sums go through LDS atomics (the 8- and 16-lane groups reduce in registers): a
+    12-point outer rule on a scalar and on the field kernel against the oracle."""
+    od = 2 if kern.endswith("Field") else 1
+    mesh = unit_square(12, 0.15, jitter=0.2, seed=5, outdim=od)
+    kernel = {"function": kern, "horizon": 0.15, "outputdim": od, "fractional_s": -0.5}
+    conf = make_configuration(close="retriangulate")
+    Px, dx = _rule12()
+    assert abs(dx.sum() - 0.5) < 1e-15
+    conf["quadrature"]["outer"] = {"points": Px, "weights": dx}
+    A, ref = _check_case(f"rule12-{kern}", mesh, kernel, conf)
+    if od == 2:
+        _check_rigid(f"rule12-{kern}", A, mesh, kernel, conf)
