@@ -123,11 +123,14 @@ int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double amin, double 
  * NLFEM_PATH_BFS_HASH: hashed BFS visited sets; NLFEM_PATH_PATTERN_HASH: hashed pattern
  * element sets; NLFEM_PATH_TRANSPOSE_CSR: transposed relation counted into exact CSR rows;
  * NLFEM_PATH_BFS_WAVE: one outer element per wavefront in the bitmap BFS (by default two
- * outer elements share a wavefront, 32 lanes each). */
+ * outer elements share a wavefront, 32 lanes each); NLFEM_PATH_TRANSPOSE_UNSTAGED: cut
+ * entries of the transposed rows placed one at a time with global atomics (otherwise taken by
+ * blocks whose cut entries exceed the LDS staging array). */
 #define NLFEM_PATH_BFS_HASH 1
 #define NLFEM_PATH_PATTERN_HASH 2
 #define NLFEM_PATH_TRANSPOSE_CSR 4
 #define NLFEM_PATH_BFS_WAVE 8
+#define NLFEM_PATH_TRANSPOSE_UNSTAGED 16
 int nlfem_plan_force_paths(nlfem_plan* plan, int flags);
 /* Cut kernel for every kernel but the constant one and for outdim 2 and DG: 0 (default) the
  * per-lane kernel (one item per lane, each wavefront on its own pairs, no chunk barriers;

@@ -910,10 +910,10 @@ __global__ __launch_bounds__(256) void k_list_extent(int nE, int cap, const uint
 // Transposed relation.  Row b lists the outer elements aT of the pairs (aT, b): pairs from
 // the front segments (interior, close) from the row's start, cut pairs from its end.  The cut
 // entries also record the pair's index in the cut-pair enumeration (coff[aT] + its position
-// in aT's back segment).  A block takes tg consecutive outer elements, whose lists share most
-// of their inner elements: per-b counts are summed in an LDS hash first, so each block issues
-// one global atomic per distinct b (with a bounded probe; keys that find no slot use global
-// atomics per entry).
+// in aT's back segment).  A block takes tg consecutive outer elements (of the tile order),
+// whose lists share most of their inner elements: per-b counts are summed in an LDS hash first,
+// so each block issues one global atomic per distinct b and segment (with a bounded probe; keys
+// that find no slot use global atomics per entry).
 constexpr int TH = 4096;
 // Insert with at most 64 probes; -1 if none free.  A key recurs up to tg times per block, so
 // each probe reads the slot first and takes the compare-and-swap only on an empty slot.
@@ -941,33 +941,17 @@ NLFEM_DEV int lds_find_b(const int* h, int key) {
     return -1;
 }
 
-// FILL = false: tcnt[b] += entries of row b.  FILL = true: place the entries; tfill / tfillB
-// count the front / back entries placed so far (they end as row b's front and cut-entry
-// counts).  Row b occupies [tptr[b], tptr[b+1]): the exact CSR rows of a counting pass, or
-// padded rows of capT entries (row b at b capT; capT = 0: CSR rows), whose overflow k_tcheck
-// reports.  PK (padded rows with capT < 32768): the block's front and back counts of a row
-// share one LDS word (16 bits each), so the tables take 32 KB instead of 48 KB; a row whose
-// counts would not fit 16 bits holds more than capT entries, which k_tcheck reports, and the
-// plan redoes the transpose with CSR rows.
-template <bool FILL, bool PK = false>
-__global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
-                                                   const int* cntB, int* tcnt, const long long* tptr, int* tfill,
-                                                   int* tadj, const long long* coff, int* tfillB, int* tcut,
-                                                   int capT, const int* ord) {
-    static_assert(!PK || FILL, "packed counts only in the fill pass");
-    __shared__ int sh[PK ? 2 * TH : 3 * TH];
-    int* hk = sh;
-    int* hf = sh + TH;
-    int* hb = PK ? hf : sh + 2 * TH;
-    constexpr int BK = PK ? 0x10000 : 1;  // one back entry in hb
+// Counting pass for exact CSR rows: tcnt[b] += entries of row b, one global atomic per distinct b
+// of the block (per entry for keys that find no LDS slot).
+__global__ __launch_bounds__(256) void k_transpose_count(int nE, int cap, int tg, const uint32_t* nbr,
+                                                         const int* cntF, const int* cntB, int* tcnt,
+                                                         const int* ord) {
+    __shared__ int hk[TH];
+    __shared__ int hf[TH];
     const int tid = threadIdx.x;
-    for (int i = tid; i < TH; i += 256) { hk[i] = -1; hf[i] = 0; hb[i] = 0; }
+    for (int i = tid; i < TH; i += 256) { hk[i] = -1; hf[i] = 0; }
     __syncthreads();
-    // the block's outer elements: tg consecutive ones of the tile order (ord), so that the
-    // blocks running at one time on one XCD write into the rows of a compact neighbourhood
     const int a0 = xcd_block(blockIdx.x, gridDim.x) * tg, a1 = min(a0 + tg, nE);
-    // entries in groups of TU per thread: the group's list loads are issued before its LDS
-    // updates, so the loads of one group overlap
     constexpr int TU = 4;
     for (int ia = a0; ia < a1; ia++) {
         const int aT = ord ? ord[ia] : ia;
@@ -985,69 +969,192 @@ __global__ __launch_bounds__(256) void k_transpose(int nE, int cap, int tg, cons
                 if (p >= n) break;
                 const int b = (int)(ev[u] & IDX_MASK);
                 const int sl = lds_slot_b(hk, b);
-                if (sl >= 0) {
-                    if (p < nF || !FILL) atomicAdd(&hf[sl], 1);
-                    else atomicAdd(&hb[sl], BK);
-                } else if (!FILL) {
-                    atomicAdd(&tcnt[b], 1);
+                if (sl >= 0) atomicAdd(&hf[sl], 1);
+                else atomicAdd(&tcnt[b], 1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < TH; i += 256)
+        if (hk[i] >= 0) atomicAdd(&tcnt[hk[i]], hf[i]);
+}
+
+// Fill pass with row runs.  Row b occupies [tptr[b], tptr[b+1]): the exact CSR rows of the
+// counting pass, or padded rows of capT entries (row b at b capT; capT = 0: CSR rows), whose
+// overflow k_tcheck reports; tfill / tfillB end as row b's front and cut-entry counts.  A block takes tg <= 8 consecutive outer elements k of the tile order.
+//   1  LDS hash of the distinct inner elements b of their lists.  Word m[b]: bit k for a front
+//      entry of outer element k that is an interior pair, bit 8 + k for a close pair, and the
+//      number of back (cut) entries in bits 16..19.
+//   2  Exclusive scan of the back counts over the slots (offsets in bits 20..31), then each
+//      back entry goes to its slot's range of an LDS array: outer element k, bit 31 of the
+//      entry, its index in k's back segment.  The slot's count is then base(next slot) - base.
+//   3  Per slot: one global atomic per segment reserves the block's ranks in row b, and the
+//      thread writes the block's front entries of the row, then its back entries, as runs of
+//      consecutive stores that merge in the L2.  (Entries placed one outer element at a time
+//      reached a row's line in passes far apart, and the lines left the L2 partially written.)
+// Entries whose key found no hash slot, unexpected classes and blocks with more back entries
+// than the LDS array holds take per-entry global atomics.
+constexpr int TS = 4000;  // back entries staged per block (offsets and indices fit 12 bits)
+static_assert(TH == 16 * 256, "the offset scan gives each thread 16 slots");
+__global__ __launch_bounds__(256) void k_transpose_fill(int nE, int cap, int tg, const uint32_t* nbr, const int* cntF,
+                                                        const int* cntB, const long long* tptr, int* tfill,
+                                                        int* tadj, const long long* coff, int* tfillB, int* tcut,
+                                                        int capT, const int* ord, int tsmax) {
+    __shared__ int hk[TH];
+    __shared__ unsigned hm[TH];
+    __shared__ unsigned short bs[TS];  // outer element k, bit 31 of the entry, index in k's back segment
+    __shared__ int sa[8], sF[8], sN[8], scan_tmp[4];
+    __shared__ long long sco[8];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < TH; i += 256) { hk[i] = -1; hm[i] = 0u; }
+    const int a0 = xcd_block(blockIdx.x, gridDim.x) * tg, nk = min(tg, nE - a0);
+    if (tid < nk) {
+        const int aT = ord ? ord[a0 + tid] : a0 + tid;
+        sa[tid] = aT;
+        sF[tid] = cntF[aT];
+        sN[tid] = sF[tid] + cntB[aT];
+        sco[tid] = coff[aT];
+    }
+    __syncthreads();
+    int nbt = 0;
+    for (int k = 0; k < nk; k++) nbt += sN[k] - sF[k];
+    const bool stage = nbt <= tsmax;  // tsmax <= TS
+    auto row_range = [&](int b, long long& r0, long long& r1) {
+        r0 = capT ? (long long)b * capT : tptr[b];
+        r1 = capT ? r0 + capT : tptr[b + 1];
+    };
+    auto is_staged_back = [](uint32_t e) { return ((e >> CLS_SHIFT) & 3u) == CLS_CUT; };
+    constexpr int TU = 4;
+    // 1: hash, front bits and back counts
+    for (int k = 0; k < nk; k++) {
+        const int aT = sa[k], nF = sF[k], n = sN[k];
+        for (int p0 = tid; p0 < n; p0 += TU * 256) {
+            uint32_t ev[TU];
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int p = p0 + u * 256;
+                ev[u] = p < n ? nbr[(size_t)aT * cap + list_index(p, nF, cap)] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int p = p0 + u * 256;
+                if (p >= n) break;
+                const uint32_t e = ev[u];
+                const int b = (int)(e & IDX_MASK);
+                if (p < nF) {
+                    const uint32_t cls = (e >> CLS_SHIFT) & 3u;
+                    const int sl = lds_slot_b(hk, b);
+                    if (sl >= 0 && !(e >> 31) && (cls == CLS_INTERIOR || cls == CLS_CLOSE)) {
+                        atomicOr(&hm[sl], 1u << (k + (cls == CLS_CLOSE ? 8 : 0)));
+                    } else {
+                        long long r0, r1;
+                        row_range(b, r0, r1);
+                        const long long r = atomicAdd(&tfill[b], 1);
+                        if (r < r1 - r0) tadj[r0 + r] = (int)((uint32_t)aT | (e & ~IDX_MASK));
+                    }
+                } else if (stage && is_staged_back(e)) {
+                    const int sl = lds_slot_b(hk, b);
+                    if (sl >= 0) atomicAdd(&hm[sl], 1u << 16);
                 }
             }
         }
     }
     __syncthreads();
-    for (int i = tid; i < TH; i += 256) {
-        const int b = hk[i];
-        if (b < 0) continue;
-        if (!FILL) atomicAdd(&tcnt[b], hf[i]);
-        else if (PK) {  // reserve the block's ranges; the word becomes the next free ranks
-            const int v = hf[i], f = v & 0xffff, k = (unsigned)v >> 16;
-            const int rf = f ? atomicAdd(&tfill[b], f) : 0;
-            const int rb = k ? atomicAdd(&tfillB[b], k) : 0;
-            hf[i] = (int)(((unsigned)rf & 0xffffu) | ((unsigned)rb << 16));
-        } else {
-            if (hf[i]) hf[i] = atomicAdd(&tfill[b], hf[i]);
-            if (hb[i]) hb[i] = atomicAdd(&tfillB[b], hb[i]);
+    // 2: offsets of the back ranges (slots 16 t .. 16 t + 15 per thread, in slot order)
+    int nstaged = 0;
+    if (stage) {
+        unsigned w[16];
+        int loc = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            w[j] = hm[16 * tid + j];
+            loc += (int)((w[j] >> 16) & 15u);
+        }
+        int off = block_exscan256(loc, scan_tmp, &nstaged);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int c = (int)((w[j] >> 16) & 15u);
+            hm[16 * tid + j] = (w[j] & 0xfffffu) | ((unsigned)off << 20);
+            off += c;
+        }
+        __syncthreads();
+    }
+    // back entries: into the slot's LDS range (staged) or to their rows (per entry)
+    for (int k = 0; k < nk; k++) {
+        const int aT = sa[k], nF = sF[k], nB = sN[k] - nF;
+        const long long co = sco[k];
+        for (int j0 = tid; j0 < nB; j0 += TU * 256) {
+            uint32_t ev[TU];
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int j = j0 + u * 256;
+                ev[u] = j < nB ? nbr[(size_t)aT * cap + cap - 1 - j] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const int j = j0 + u * 256;
+                if (j >= nB) break;
+                const uint32_t e = ev[u];
+                const int b = (int)(e & IDX_MASK);
+                const int sl = stage && is_staged_back(e) ? lds_find_b(hk, b) : -1;
+                if (sl >= 0) {
+                    const unsigned old = atomicSub(&hm[sl], 1u << 16);
+                    bs[(old >> 20) + ((old >> 16) & 15u) - 1u] = (unsigned short)((unsigned)k | ((e >> 31) << 3) | ((unsigned)j << 4));
+                } else {
+                    long long r0, r1;
+                    row_range(b, r0, r1);
+                    const long long r = atomicAdd(&tfillB[b], 1);
+                    if (r >= r1 - r0) continue;
+                    const long long q = r1 - 1 - r;
+                    tcut[q] = (int)(co + j);
+                    tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));
+                }
+            }
         }
     }
-    if (!FILL) return;
     __syncthreads();
-    for (int ia = a0; ia < a1; ia++) {
-        const int aT = ord ? ord[ia] : ia;
-        const int nF = cntF[aT], n = nF + cntB[aT];
-        const long long co = coff[aT];
-        for (int p0 = tid; p0 < n; p0 += TU * 256) {
-          uint32_t ev[TU];
+    // 3: reserve and write the runs, RU slots per thread at a time (their atomics in flight
+    // together)
+    constexpr int RU = 4;
+    for (int i0 = tid; i0 < TH; i0 += RU * 256) {
+        int bb[RU], rf[RU], rb[RU];
 #pragma unroll
-          for (int u = 0; u < TU; u++) {
-              const int p = p0 + u * 256;
-              ev[u] = p < n ? nbr[(size_t)aT * cap + list_index(p, nF, cap)] : 0u;
-          }
+        for (int u = 0; u < RU; u++) {
+            const int i = i0 + u * 256;
+            bb[u] = hk[i];
+            rf[u] = rb[u] = 0;
+            if (bb[u] < 0) continue;
+            const unsigned w = hm[i];
+            const int f = __popc(w & 0xffffu);
+            const int kb = stage ? (i + 1 < TH ? (int)(hm[i + 1] >> 20) : nstaged) - (int)(w >> 20) : 0;
+            if (f) rf[u] = atomicAdd(&tfill[bb[u]], f);
+            if (kb) rb[u] = atomicAdd(&tfillB[bb[u]], kb);
+        }
 #pragma unroll
-          for (int u = 0; u < TU; u++) {
-            const int p = p0 + u * 256;
-            if (p >= n) break;
-            const uint32_t e = ev[u];
-            const int b = (int)(e & IDX_MASK);
-            const int sl = lds_find_b(hk, b);
-            long long q;
-            const long long r0 = capT ? (long long)b * capT : tptr[b];
-            const long long r1 = capT ? r0 + capT : tptr[b + 1];
-            const long long room = r1 - r0;  // the row's capacity
-            if (p < nF) {
-                const long long r = sl >= 0 ? (long long)(atomicAdd(&hf[sl], 1) & (PK ? 0xffff : -1))
-                                            : atomicAdd(&tfill[b], 1);
-                if (r >= room) continue;  // padded rows: overflow, detected by k_tcheck
-                q = r0 + r;
-            } else {
-                const long long r = sl >= 0 ? (long long)(PK ? ((unsigned)atomicAdd(&hb[sl], BK) >> 16)
-                                                              : (unsigned)atomicAdd(&hb[sl], 1))
-                                            : atomicAdd(&tfillB[b], 1);
-                if (r >= room) continue;
-                q = r1 - 1 - r;
-                tcut[q] = (int)(co + (p - nF));
+        for (int u = 0; u < RU; u++) {
+            const int i = i0 + u * 256, b = bb[u];
+            if (b < 0) continue;
+            const unsigned w = hm[i], mf = w & 0xffffu;
+            const int base = (int)(w >> 20);
+            const int kb = stage ? (i + 1 < TH ? (int)(hm[i + 1] >> 20) : nstaged) - base : 0;
+            long long r0, r1;
+            row_range(b, r0, r1);
+            const long long room = r1 - r0;
+            unsigned mm = (mf | (mf >> 8)) & 0xffu;
+            for (long long r = rf[u]; mm; mm &= mm - 1u, r++) {
+                const int k = __ffs(mm) - 1;
+                const uint32_t cls = (mf >> (8 + k)) & 1u ? CLS_CLOSE : CLS_INTERIOR;
+                if (r < room) tadj[r0 + r] = (int)((uint32_t)sa[k] | (cls << CLS_SHIFT));
             }
-            tadj[q] = (int)((uint32_t)aT | (e & ~IDX_MASK));  // outer element + pair class bits
-          }
+            for (int c = 0; c < kb; c++) {
+                const long long r = (long long)rb[u] + c;
+                if (r >= room) break;
+                const unsigned v = bs[base + c];
+                const int k = (int)(v & 7u);
+                const long long q = r1 - 1 - r;
+                tcut[q] = (int)(sco[k] + (v >> 4));
+                tadj[q] = (int)((uint32_t)sa[k] | ((v >> 3) & 1u) << 31);
+            }
         }
     }
 }
@@ -4833,8 +4940,9 @@ void plan_assemble(Plan& P, const Estimates& est) {
     // outer elements per transpose block: with a local element numbering (the BFS stayed in its
     // index window) consecutive elements share most of their lists, and eight lists of up to
     // 1400 entries keep the LDS hash under half full
-    const int tg = (use_bm && P.stats.max_list <= 1400) ? 8 : 1;
+    const int tg = (use_bm && P.stats.max_list <= 1400) ? 8 : 1;  // k_transpose_fill: at most 8
     const int tblocks = (c.nE + tg - 1) / tg;
+    const int tsmax = P.tunstaged ? -1 : TS;
     dreserve(P, P.d_tptr, (size_t)c.nE + 1);
     dreserve(P, P.d_coff, (size_t)c.nE + 1);
     dreserve(P, P.d_tcB, (size_t)c.nE);
@@ -4850,14 +4958,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
         k_tptr_padded<<<nblk(c.nE + 1, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tptr);
         HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
         HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
-        if (P.cap < 32768)
-            k_transpose<true, true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
-                                                             P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut,
-                                                             P.cap, P.d_ord);
-        else
-            k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr,
-                                                       P.d_tptr, P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, P.cap,
-                                                       P.d_ord);
+        k_transpose_fill<<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, P.d_tF,
+                                                  P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, P.cap, P.d_ord, tsmax);
         k_tcheck<<<nblk(c.nE, 256), 256, 0, st>>>(c.nE, P.cap, P.d_tF, P.d_tcB, P.d_counters);
         HIPCHECK(hipGetLastError());
         read_counters(P, h);
@@ -4871,8 +4973,7 @@ void plan_assemble(Plan& P, const Estimates& est) {
     }
     if (!tdone) {
         HIPCHECK(hipMemsetAsync(d_cnt, 0, sizeof(int) * (size_t)c.nE, st));
-        k_transpose<false><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, nullptr,
-                                                    nullptr, nullptr, nullptr, nullptr, nullptr, 0, P.d_ord);
+        k_transpose_count<<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, d_cnt, P.d_ord);
         scan_exclusive<int>(d_cnt, P.d_tptr, c.nE, scan_scratch(P), st);
         long long ntadj = 0;
         HIPCHECK(hipMemcpyAsync(&ntadj, P.d_tptr + c.nE, sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -4881,8 +4982,8 @@ void plan_assemble(Plan& P, const Estimates& est) {
         dreserve(P, P.d_tcut, (size_t)ntadj);
         HIPCHECK(hipMemsetAsync(P.d_tF, 0, sizeof(int) * (size_t)c.nE, st));
         HIPCHECK(hipMemsetAsync(P.d_tcB, 0, sizeof(int) * (size_t)c.nE, st));
-        k_transpose<true><<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, nullptr, P.d_tptr,
-                                                   P.d_tF, P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, 0, P.d_ord);
+        k_transpose_fill<<<tblocks, 256, 0, st>>>(c.nE, P.cap, tg, P.d_nbr, P.d_cntF, P.d_cntB, P.d_tptr, P.d_tF,
+                                                  P.d_tadj, P.d_coff, P.d_tcB, P.d_tcut, 0, P.d_ord, tsmax);
     }
     HIPCHECK(hipGetLastError());
     ev_end(P, PH_TRANSPOSE);

@@ -407,11 +407,12 @@ extern "C" int nlfem_plan_set_estimates(nlfem_plan* plan, double hmax, double am
 
 extern "C" int nlfem_plan_force_paths(nlfem_plan* plan, int flags) {
     if (!plan) return fail(NLFEM_EINVAL, "null plan");
-    if (flags & ~15) return fail(NLFEM_EINVAL, "unknown path flags");
+    if (flags & ~31) return fail(NLFEM_EINVAL, "unknown path flags");
     if (flags & NLFEM_PATH_BFS_WAVE) plan->P.bfs_wave = true;
     if (flags & NLFEM_PATH_BFS_HASH) plan->P.bfs_nobm = true;
     if (flags & NLFEM_PATH_PATTERN_HASH) plan->P.pat_nobm = true;
     if (flags & NLFEM_PATH_TRANSPOSE_CSR) plan->P.tpad_off = true;
+    if (flags & NLFEM_PATH_TRANSPOSE_UNSTAGED) plan->P.tunstaged = true;
     return NLFEM_OK;
 }
 

@@ -104,6 +104,7 @@ struct Plan {
     int* d_tcB = nullptr;         // [nE] cut entries at the end of each transposed row
     int* d_tF = nullptr;          // [nE] interior and close entries at the start of each row
     bool tpad_off = false;        // a padded transposed row overflowed once: count first
+    bool tunstaged = false;       // cut entries of the transposed rows placed one at a time
     int* d_tcut = nullptr;        // [ntadj] cut entries: index of the pair in the cut enumeration
     double* d_stage = nullptr;    // [ncut*6] termLocalPrime per cut pair
     long long stage_cap = 0;

@@ -117,12 +117,15 @@ class Assembler:
             self.set_surrogate(**(sur if isinstance(sur, dict) else {}))
 
     PATH_BFS_HASH, PATH_PATTERN_HASH, PATH_TRANSPOSE_CSR, PATH_BFS_WAVE = 1, 2, 4, 8
+    PATH_TRANSPOSE_UNSTAGED = 16
 
     def force_paths(self, flags):
         """Use the general data structures (hashed BFS visited sets, hashed pattern element
         sets, counted CSR transposed rows) that are otherwise taken only for poorly numbered
         meshes or after a fixed-size layout overflowed; PATH_BFS_WAVE runs the bitmap BFS with
-        one outer element per wavefront instead of two."""
+        one outer element per wavefront instead of two; PATH_TRANSPOSE_UNSTAGED places the cut
+        entries of the transposed rows one at a time (the path of blocks whose cut entries do
+        not fit the LDS staging array)."""
         check(lib().nlfem_plan_force_paths(self._h, int(flags)))
         return self
 

@@ -296,8 +296,10 @@ def test_parity_shuffled_element_numbering():
 
 @pytest.mark.gpu
 def test_parity_general_paths():
-    """The hashed BFS visited sets, hashed pattern element sets and counted CSR transposed rows,
-    forced on a locally numbered mesh, give the same lists, pattern and values."""
+    """The hashed BFS visited sets, hashed pattern element sets, counted CSR transposed rows and
+    cut entries of the transposed rows placed one at a time (also on baryCenter pairs, whose
+    entries carry bit 31), forced on a locally numbered mesh, give the same lists, pattern and
+    values."""
     from nlfem_amd.mesh import unit_square
     from nlfem_amd.config import make_configuration
     m = unit_square(40, 0.1, jitter=0.15, seed=5)
@@ -308,6 +310,11 @@ def test_parity_general_paths():
                 paths=nlfem.Assembler.PATH_TRANSPOSE_CSR)
     _check_case("bfs-wave", m, {"function": "constant", "horizon": 0.1, "outputdim": 1}, make_configuration(),
                 paths=nlfem.Assembler.PATH_BFS_WAVE)
+    _check_case("transpose-unstaged", m, {"function": "constant", "horizon": 0.1, "outputdim": 1},
+                make_configuration(), paths=nlfem.Assembler.PATH_TRANSPOSE_UNSTAGED)
+    mb, kb, cb = case("bary")  # baryCenter pairs: cut entries with bit 31
+    _check_case("bary-transpose-unstaged", mb, kb, cb, paths=nlfem.Assembler.PATH_TRANSPOSE_UNSTAGED)
+    _check_case("bary-transpose-csr", mb, kb, cb, paths=nlfem.Assembler.PATH_TRANSPOSE_CSR)
 
 
 @pytest.mark.parametrize("name", ["C5s", "DG", "C2s"])
