@@ -2181,11 +2181,158 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
         rlen[j] = (int)(a.rowptr[rA[j] + 1] - rp[j]);
     }
     const ColMap cm = colmap_build(hk, a.HN, a.col, rp, rlen, labA, scan_tmp, a.counters);
-    // NPY > 0: the general (non-factorised) kernels with an inner rule of NPY points
-    constexpr bool FIXED = NPY > 0 && !(KER == KER_CONSTANT && OD == 1);
+    // the scalar parabola kernel: moment sums (PFACT); NPY > 0: the other general kernels with an
+    // inner rule of NPY points
+    constexpr bool PFACT = KER == KER_PARABOLA && OD == 1;
+    constexpr bool FIXED = NPY > 0 && !(KER == KER_CONSTANT && OD == 1) && !PFACT;
     const int nPx = a.q.nPx;
     const int lane = tid & 63, w = tid >> 6;
-    if constexpr (FIXED) {
+    if constexpr (PFACT) {
+        // Parabola kernel K(x, y) = c (delta^2 - |x - y|^2) (src/model.cpp:52-60).  With x = o + xt
+        // and y = o + yt about T's first vertex o, a rule sum of weights f(x_k) g(y_l) K(x_k, y_l)
+        // over the outer points x_k and inner points y_l (src/integration.cpp:508-537) is
+        //   c [delta^2 F0 G0 - F2 G0 + 2 F1 . G1 - F0 G2],
+        // with F0, F1, F2 = sum_k f(x_k) (1, xt_k, |xt_k|^2) and G likewise over the inner points.
+        // The moments of T's own points are formed once per block, the partner's per pair: about
+        // 300 flop per pair and role instead of 49 kernel evaluations.  In FP64 for both
+        // precisions (every term is of the size of delta^2 F0 G0; an interior pair keeps
+        // |x - y| below delta, so no sum cancels by more than a few digits).
+        __shared__ double FT[9][4];  // T as outer element: f = |T| dx_k psi_i psi_j (x_k) (sym6), |T| dx_k psi_i (x_k) (6 + i)
+        __shared__ double GT[9][4];  // T as inner element: g = |T| dy_l psi_a psi_b (y_l) (sym6), |T| dy_l psi_j (y_l) (6 + j)
+        const double o0 = A.E[0], o1 = A.E[1];
+        const double c = a.kp.c, d2 = a.kp.sqdelta;
+        auto combine = [&](const double* F, const double* G) {
+            return c * (d2 * F[0] * G[0] - F[3] * G[0] + 2.0 * (F[1] * G[1] + F[2] * G[2]) - F[0] * G[3]);
+        };
+        if (tid < 18) {
+            const bool outer = tid < 9;
+            const int f = outer ? tid : tid - 9;
+            const int np = outer ? nPx : a.q.nPy;
+            // basis indices of the function: (i, j) with sym6(i, j) = f, or psi_{f - 6}
+            int bi = 0, bj = 0;
+            if (f < 6)
+                for (int i = 0; i < 3; i++)
+                    for (int j = i; j < 3; j++)
+                        if (sym6(i, j) == f) { bi = i; bj = j; }
+            double m[4] = {0, 0, 0, 0};
+            for (int k = 0; k < np; k++) {
+                const double* P = outer ? a.q.Px : a.q.Py;
+                const double* ps = outer ? a.q.psix : a.q.psiy;
+                double x[2];
+                toPhys(A.E, P[2 * k], P[2 * k + 1], x);
+                const double t0 = x[0] - o0, t1 = x[1] - o1;
+                const double wgt = A.absDet * (outer ? a.q.dx[k] : a.q.dy[k]) *
+                                   (f < 6 ? ps[3 * k + bi] * ps[3 * k + bj] : ps[3 * k + f - 6]);
+                m[0] += wgt;
+                m[1] += wgt * t0;
+                m[2] += wgt * t1;
+                m[3] += wgt * (t0 * t0 + t1 * t1);
+            }
+            double* dst = outer ? FT[f] : GT[f];
+            for (int e = 0; e < 4; e++) dst[e] = m[e];
+        }
+        __syncthreads();
+        // T as outer element: termNonloc into (T_i, U_j) per pair; termLocal (six symmetric sums)
+        // is linear in the partners' moments of g = |U| dy_l, summed over the pairs (SG)
+        double SG[4] = {0, 0, 0, 0};
+        const int nF = a.cntF[T0];
+        for (int p = tid; p < nF; p += 256) {
+            const uint32_t e = a.nbr[(size_t)T0 * a.cap + p];
+            if (((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const ElemRec B = a.elem[U];
+            double G[4][4];  // g = |U| dy_l psi_j (y_l) (G[j]); G[3]: g = |U| dy_l
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) G[s][t] = 0;
+            for (int l = 0; l < a.q.nPy; l++) {
+                double y[2];
+                toPhys(B.E, a.q.Py[2 * l], a.q.Py[2 * l + 1], y);
+                const double t0 = y[0] - o0, t1 = y[1] - o1, r2 = t0 * t0 + t1 * t1;
+                const double v = a.q.dy[l] * B.absDet;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const double g = s == 3 ? v : v * a.q.psiy[3 * l + s];
+                    G[s][0] += g;
+                    G[s][1] += g * t0;
+                    G[s][2] += g * t1;
+                    G[s][3] += g * r2;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) SG[t] += G[3][t];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int s = cm.find(node_of(a.tri, U, j, a.is_DG));
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if (!labA[i]) continue;
+                    if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+                    atomicAdd(&acc[s * 3 + i], -combine(FT[6 + i], G[j]));
+                }
+            }
+        }
+        // T as inner element: pairs (U, T).  termLocalPrime is linear in U's weight moments
+        // W = sum_k |U| dx_k (1, xt_k, |xt_k|^2), summed over the pairs; termNonlocPrime into
+        // (T_j, U_i) per pair from F = sum_k |U| dx_k psi_i (x_k) (1, xt_k, |xt_k|^2)
+        double W[4] = {0, 0, 0, 0};
+        const long long tb = a.tptr[T0], te = tb + a.tF[T0];
+        for (long long q = tb + tid; q < te; q += 256) {
+            const uint32_t e = (uint32_t)a.tadj[q];
+            if ((e >> 31) || ((e >> CLS_SHIFT) & 3u) != CLS_INTERIOR) continue;
+            const int U = (int)(e & IDX_MASK);
+            const ElemRec B = a.elem[U];
+            double F[3][4];
+#pragma unroll
+            for (int s = 0; s < 3; s++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) F[s][t] = 0;
+            for (int k = 0; k < nPx; k++) {
+                double x[2];
+                toPhys(B.E, a.q.Px[2 * k], a.q.Px[2 * k + 1], x);
+                const double t0 = x[0] - o0, t1 = x[1] - o1, r2 = t0 * t0 + t1 * t1;
+                const double wk = a.q.dx[k] * B.absDet;
+                W[0] += wk;
+                W[1] += wk * t0;
+                W[2] += wk * t1;
+                W[3] += wk * r2;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const double f = wk * a.q.psix[3 * k + i];
+                    F[i][0] += f;
+                    F[i][1] += f * t0;
+                    F[i][2] += f * t1;
+                    F[i][3] += f * r2;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int s = cm.find(node_of(a.tri, U, i, a.is_DG));
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    if (!labA[j]) continue;
+                    if (s < 0) { atomicOr(&a.counters[CNT_ERROR], ERR_HASH); continue; }
+                    atomicAdd(&acc[s * 3 + j], -combine(F[i], GT[6 + j]));
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            for (int m = 32; m > 0; m >>= 1) {
+                SG[t] += shfl_xor_d(SG[t], m);
+                W[t] += shfl_xor_d(W[t], m);
+            }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    red[w][3 * i + j] = combine(FT[sym6(i, j)], SG);
+                    red[w][NB + 3 * i + j] = combine(W, GT[sym6(i, j)]);
+                }
+        }
+    } else if constexpr (FIXED) {
         // Inner rule of NPY points unrolled.  Per block: the outer points xo and inner points yi
         // of T, and the outer weights wxa[k][i] = |T| dx_k psi_i(x_k), wl[k][sym(i,j)] =
         // |T| dx_k psi_i(x_k) psi_j(x_k) that outer_point applies.
