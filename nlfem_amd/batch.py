@@ -115,8 +115,10 @@ class BatchAssembler:
 
 def c5_meshes(count=48, seed=0, max_ratio=12.0, outdim=2):
     """Batched uneven workload: h from {1/32, 1/64, 1/128, 1/256}, jitter +-0.2h, delta uniform in
-    [0.05, 0.2] limited to delta <= max_ratio * h (the largest horizon, in elements, that the
-    BFS and pattern LDS tables of this build hold)."""
+    [0.05, 0.2] limited to delta <= max_ratio * h.  One plan gives every outer element the
+    neighbour-list capacity of the mesh with the most elements per horizon; at delta = 0.2 on
+    h = 1/256 (51 h) that is ~16,000 slots for each of the ~1.9 M elements (~120 GB for the
+    lists, as much for their transpose), so the workload keeps the horizons at 12 h."""
     from .mesh import unit_square
     rng = np.random.default_rng(seed)
     out = []
