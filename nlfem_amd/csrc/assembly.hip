@@ -330,8 +330,11 @@ NLFEM_DEV void outer_points(const BfsArgs& a, const ElemRec& A, OuterPts& op) {
 
 // Level-0 method codes: M_RETRIANGULATE, M_BARYCENTER, or 2 = tensor Gauss (always interacts).
 // classify_quick returns the pair class, -1 when the pair does not interact, or CLS_HARD when
-// only retriangulations can tell (classify_full).  The conservative tests use margins (1e-7,
-// 1e-6, 1e-9 relative) far above rounding, so they only ever agree with the exact test:
+// only retriangulations can tell (the exact test of the hard candidates in k_bfs and k_bfs_h:
+// some outer point's polygon has >= 3 points, skipping an outer point whose ball stays
+// farther than delta (1 + 1e-9) from all three edges of bT, which has no polygon point).  The
+// conservative tests use margins (1e-7, 1e-6, 1e-9 relative) far above rounding, so they only
+// ever agree with the exact test:
 //  - every vertex of bT within delta(1-1e-7) - r of the outer points' centre: interior;
 //  - the bounding circle of bT farther than delta(1+1e-6) + r: every vertex is outside every
 //    ball and no edge reaches one, so the reference's polygon is empty for every k;
@@ -389,19 +392,6 @@ NLFEM_DEV bool tri_far(double x0, double x1, const double* E, double sqf) {
            seg_far(x0, x1, E[4], E[5], E[0], E[1], sqf);
 }
 
-// The reference predicate (some outer point's polygon has >= 3 points) by retriangulation.
-// An outer point whose ball stays farther than delta (1 + 1e-9) from all three edges of bT
-// has no polygon point and is skipped.
-NLFEM_DEV int classify_full(const BfsArgs& a, const OuterPts& op, const ElemRec& B) {
-    for (int k = 0; k < a.q.nPx; k++) {
-        const double x0 = op.x[2 * k], x1 = op.x[2 * k + 1];
-        if (tri_far(x0, x1, B.E, a.sq_far)) continue;
-        NoEmit ne;
-        if (retriangulate<true>(x0, x1, B.E, a.sqdelta, a.caps != 0, ne) >= 3) return (int)CLS_CUT;
-    }
-    return a.fullCC ? (int)CLS_CUT : -1;
-}
-
 // Block index with the XCD-major order: workgroups are dispatched to the 8 XCDs round-robin
 // (hardware block i runs on XCD i % 8), so the hardware blocks of one XCD take one contiguous
 // range of logical blocks.  Neighbouring outer elements write into the same rows of shared
@@ -448,13 +438,14 @@ NLFEM_DEV int visit_insert(int* vs, const BfsArgs& a, int aT, int e) {
 // {aT} u adj(aT) uses the close method.  The resulting set does not depend on visit order.
 // All counters are wavefront-uniform registers, appended to with ballot ranks (no LDS atomics);
 // the candidates that only a retriangulation can settle are compacted in place and tested
-// densely after the cheap tests of the level.
+// densely after the cheap tests of the level, one lane per (candidate, outer point).
 template <bool BM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_bfs(BfsArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs(BfsArgs a) {
     extern __shared__ int lds[];
     int* hash = lds;
     int* test = hash + a.H;
     int* next = test + a.F;
+    unsigned* hflag = reinterpret_cast<unsigned*>(next + a.F);  // one bit per hard candidate
     __shared__ OuterPts op;
     const int lane = threadIdx.x;
     const int aT = blockIdx.x;
@@ -550,16 +541,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             record(b, cls, push);
         }
         __syncthreads();
-        for (int r0 = 0; r0 < nHard; r0 += 64) {
-            const int j = r0 + lane;
-            const bool valid = j < nHard;
-            const int b = valid ? test[j] : -1;
-            int cls = -1;
-            if (valid) {
-                cls = classify_full(a, op, a.elem[b]);
-                cProbe += cls < 0;
+        if (nHard > 0) {
+            // the exact test (the reference predicate: some outer point's polygon has >= 3
+            // points) with one lane per (hard candidate, outer point), OR-ed into the
+            // candidate's bit
+            const int nPx = a.q.nPx;
+            for (int i = lane; i <= nHard / 32; i += 64) hflag[i] = 0u;
+            __syncthreads();
+            for (int it = lane; it < nHard * nPx; it += 64) {
+                const int j = it / nPx, k = it - j * nPx;
+                const ElemRec B = a.elem[test[j]];
+                const double x0 = op.x[2 * k], x1 = op.x[2 * k + 1];
+                if (!tri_far(x0, x1, B.E, a.sq_far)) {
+                    NoEmit ne;
+                    if (retriangulate<true>(x0, x1, B.E, a.sqdelta, a.caps != 0, ne) >= 3)
+                        atomicOr(&hflag[j >> 5], 1u << (j & 31));
+                }
             }
-            record(b, cls, cls >= 0);
+            __syncthreads();
+            for (int r0 = 0; r0 < nHard; r0 += 64) {
+                const int j = r0 + lane;
+                const bool valid = j < nHard;
+                const int b = valid ? test[j] : -1;
+                int cls = -1;
+                if (valid) {
+                    cls = (((hflag[j >> 5] >> (j & 31)) & 1u) || a.fullCC) ? (int)CLS_CUT : -1;
+                    cProbe += cls < 0;
+                }
+                record(b, cls, cls >= 0);
+            }
         }
         ovf = __any(ovf);
         __syncthreads();
@@ -686,14 +696,18 @@ NLFEM_DEV int half_rank(bool p, int& total) {
 NLFEM_DEV bool half_any(bool p) { return (__ballot(p) & half_mask()) != 0ull; }
 NLFEM_DEV int wave_max2(int v) { return max(__shfl(v, 0, 64), __shfl(v, 32, 64)); }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_bfs_h(BfsArgs a) {
+// Capped at 128 registers (4 wavefronts per SIMD, 14 spilled): the exact test of the hard
+// candidates runs one outer point per lane, which keeps its registers low enough that the
+// extra occupancy pays (C2 search 1.47 -> 1.33 ms, C4 unchanged).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_h(BfsArgs a) {
     extern __shared__ int ldsh[];
     __shared__ OuterPts ops[2];
     const int lane = threadIdx.x;
     const int hf = lane >> 5, hl = lane & 31;
-    int* hash = ldsh + hf * (a.H + a.F);
+    int* hash = ldsh + hf * (a.H + a.F + a.F / 32 + 1);
     unsigned short* test = reinterpret_cast<unsigned short*>(hash + a.H);
     unsigned short* next = test + a.F;
+    unsigned* hflag = reinterpret_cast<unsigned*>(hash + a.H + a.F);  // one bit per hard candidate
     OuterPts& op = ops[hf];
     const int slot = 2 * blockIdx.x + hf;  // the two outer elements: consecutive ones of the tile order
     const bool live = slot < a.nE;
@@ -790,16 +804,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }
         __syncthreads();
         const int mH = wave_max2(nHard);
-        for (int r0 = 0; r0 < mH; r0 += 32) {
-            const int j = r0 + hl;
-            const bool valid = j < nHard;
-            const int b = valid ? base + (int)test[j] : -1;
-            int cls = -1;
-            if (valid) {
-                cls = classify_full(a, op, a.elem[b]);
-                cProbe += cls < 0;
+        if (mH > 0) {
+            // the exact test (the reference predicate: some outer point's polygon has >= 3
+            // points) with one lane per (hard candidate, outer point): each lane runs one
+            // retriangulation and ORs a hit into its candidate's bit, instead of one lane per
+            // candidate running up to nPx of them one after another
+            const int nPx = a.q.nPx;
+            for (int i = hl; i <= nHard / 32; i += 32) hflag[i] = 0u;
+            __syncthreads();
+            const int nItems = nHard * nPx;
+            const int mI = wave_max2(nItems);
+            for (int r0 = 0; r0 < mI; r0 += 32) {
+                const int it = r0 + hl;
+                if (it < nItems) {
+                    const int j = it / nPx, k = it - j * nPx;
+                    const ElemRec B = a.elem[base + (int)test[j]];
+                    const double x0 = op.x[2 * k], x1 = op.x[2 * k + 1];
+                    if (!tri_far(x0, x1, B.E, a.sq_far)) {
+                        NoEmit ne;
+                        if (retriangulate<true>(x0, x1, B.E, a.sqdelta, a.caps != 0, ne) >= 3)
+                            atomicOr(&hflag[j >> 5], 1u << (j & 31));
+                    }
+                }
             }
-            record(b, cls, cls >= 0);
+            __syncthreads();
+            for (int r0 = 0; r0 < mH; r0 += 32) {
+                const int j = r0 + hl;
+                const bool valid = j < nHard;
+                const int b = valid ? base + (int)test[j] : -1;
+                int cls = -1;
+                if (valid) {
+                    cls = (((hflag[j >> 5] >> (j & 31)) & 1u) || a.fullCC) ? (int)CLS_CUT : -1;
+                    cProbe += cls < 0;
+                }
+                record(b, cls, cls >= 0);
+            }
         }
         ovf = half_any(ovf);
         __syncthreads();
@@ -5017,13 +5056,16 @@ void plan_assemble(Plan& P, const Estimates& est) {
         // two outer elements per wavefront (k_bfs_h) with the bitmap when both searches' tables
         // stay small
         // (16-bit level buffers: a window of at most 65,536 elements)
-        bool half = use_bm && !P.bfs_wave && 32 * H <= 65536 && 2 * sizeof(int) * (size_t)(H + F) <= 64 * 1024;
-        size_t ldsb = half ? 2 * sizeof(int) * (size_t)(H + F) : sizeof(int) * (size_t)(H + 2 * F);
+        // (k_bfs_h: per search the visited bitmap, two level buffers of F 16-bit offsets and a bit
+        // per hard candidate)
+        const size_t hwords = (size_t)(H + F + F / 32 + 1);
+        bool half = use_bm && !P.bfs_wave && 32 * H <= 65536 && 2 * sizeof(int) * hwords <= 64 * 1024;
+        size_t ldsb = half ? 2 * sizeof(int) * hwords : sizeof(int) * (size_t)(H + 2 * F + F / 32 + 1);
         if (ldsb > 160 * 1024) {
             // too large for LDS: fall back to the largest table that fits and rely on overflow detection
             H = use_bm ? H : 16384;
             F = std::min<long long>(F, 8192);
-            ldsb = sizeof(int) * (size_t)(H + 2 * F);
+            ldsb = sizeof(int) * (size_t)(H + 2 * F + F / 32 + 1);
         }
         ba.whalf = (int)whalf;
         P.cap = (int)cap;
