@@ -698,7 +698,8 @@ NLFEM_DEV int wave_max2(int v) { return max(__shfl(v, 0, 64), __shfl(v, 32, 64))
 
 // Capped at 128 registers (4 wavefronts per SIMD, 14 spilled): the exact test of the hard
 // candidates runs one outer point per lane, which keeps its registers low enough that the
-// extra occupancy pays (C2 search 1.47 -> 1.33 ms, C4 unchanged).
+// extra occupancy pays (jittered h = 1/128 mesh, delta = 0.1: 1.47 -> 1.33 ms; h = 1/256,
+// delta = 0.05: unchanged).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_h(BfsArgs a) {
     extern __shared__ int ldsh[];
     __shared__ OuterPts ops[2];
@@ -1273,14 +1274,27 @@ NLFEM_DEV void pattern_row(const PatArgs& a, const int r, int* eh, int* dh) {
         const long long tb = a.tptr[T], te = a.tptr[T + 1];
         const int ntF = a.tF[T], nt = ntF + a.tB[T];
         if (n + nt > 0 && tid == 0) einsert(T);
-        for (int p = tid; p < n + nt; p += 256) {
-            int U;
-            if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
-            else {
-                const int e = p - n;
-                U = (int)((uint32_t)a.tadj[e < ntF ? tb + e : te - 1 - (e - ntF)] & IDX_MASK);
+        // the entries in groups of PU per thread, their loads issued before the inserts (the
+        // loop is bound by the latency of these loads: 2.63 -> 2.47 ms on the h = 1/256 unit
+        // square with delta = 0.05; 8 per group 2.62 ms)
+        const int tot = n + nt;
+        constexpr int PU = 4;
+        for (int p0 = tid; p0 < tot; p0 += PU * 256) {
+            int Us[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const int p = p0 + u * 256;
+                int U = -1;
+                if (p < n) U = (int)(a.nbr[(size_t)T * a.cap + list_index(p, nF, a.cap)] & IDX_MASK);
+                else if (p < tot) {
+                    const int e = p - n;
+                    U = (int)((uint32_t)a.tadj[e < ntF ? tb + e : te - 1 - (e - ntF)] & IDX_MASK);
+                }
+                Us[u] = U;
             }
-            einsert(U);
+#pragma unroll
+            for (int u = 0; u < PU; u++)
+                if (Us[u] >= 0) einsert(Us[u]);
         }
     }
     __syncthreads();
