@@ -1711,6 +1711,11 @@ struct FanBasis {
         a0 = q0[0]; b0 = q0[1] - q0[0]; c0 = q0[2] - q0[0];
         a1 = q1[0]; b1 = q1[1] - q1[0]; c1 = q1[2] - q1[0];
     }
+    // from the corners' reference coordinates (q0[c], q1[c]) = toRef(corner c), already solved
+    NLFEM_DEV FanBasis(double q00, double q10, double q01, double q11, double q02, double q12) {
+        a0 = q00; b0 = q01 - q00; c0 = q02 - q00;
+        a1 = q10; b1 = q11 - q10; c1 = q12 - q10;
+    }
     NLFEM_DEV void at(double P0, double P1, double& p0, double& p1) const {
         p0 = fma(P1, c0, fma(P0, b0, a0));
         p1 = fma(P1, c1, fma(P0, b1, a1));
@@ -3500,13 +3505,24 @@ __global__ __launch_bounds__(256, (OD == 2 && sizeof(T) == 8) ? (GSZ == 16 ? 2 :
         for (int o = 0; o < 6 * OO; o++) sLP[o] = 0;
         if (mt > 0) {
             const PivotLU lu = pivot_lu(B.E);
+            // the fan shares its apex R0, and triangle t's second corner is triangle t - 1's third:
+            // one toRef per triangle (the same solves, so the same values as three per triangle)
             const double2 R0 = col[0];
+            double2 R1 = col[256];
+            double q00, q10, q01, q11;
+            lu.solve(R0.x, R0.y, q00, q10);
+            lu.solve(R1.x, R1.y, q01, q11);
             for (int t = 0; t < mt; t++) {
                 if (t < nt) {
-                    const double2 R1 = col[(t + 1) * 256], R2 = col[(t + 2) * 256];
+                    const double2 R2 = col[(t + 2) * 256];
+                    double q02, q12;
+                    lu.solve(R2.x, R2.y, q02, q12);
                     const double rT[6] = {R0.x, R0.y, R1.x, R1.y, R2.x, R2.y};
                     const double rTdet = absDet(rT);
-                    const FanBasis fb(lu, rT);
+                    const FanBasis fb(q00, q10, q01, q11, q02, q12);
+                    R1 = R2;
+                    q01 = q02;
+                    q11 = q12;
                     for (int i = 0; i < a.q.nPy; i++) {
                         double y[2];
                         toPhys(rT, a.q.Py[2 * i], a.q.Py[2 * i + 1], y);
