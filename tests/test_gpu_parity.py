@@ -407,3 +407,37 @@ def test_parity_outer_rule_12_points(kern):
     A, ref = _check_case(f"rule12-{kern}", mesh, kernel, conf)
     if od == 2:
         _check_rigid(f"rule12-{kern}", A, mesh, kernel, conf)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,delta,kern,jitter,ckw", [
+    (256, 0.05, {"function": "constant", "horizon": 0.05, "outputdim": 1}, 0.0, {}),
+    (128, 0.1, {"function": "parabola", "horizon": 0.1, "outputdim": 1}, 0.2, {}),
+    (64, 0.2, {"function": "fractional", "horizon": 0.2, "outputdim": 1, "fractional_s": 0.4}, 0.0,
+     {"close": "fractional", "tensorGaussDegree": 5}),
+], ids=["h256_constant", "h128_parabola_jitter", "h64_fractional"])
+def test_full_size_invariants(n, delta, kern, jitter, ckw):
+    """At the full sizes of the single-mesh workloads (the oracle takes 4 to 40 s on them; the
+    bench's --vs-oracle runs compare them entry by entry): A 1 = 0 on every assembled row and a
+    symmetric Omega x Omega block to 1e-13 of max|A| (an entry without its mirror counts with its
+    whole value: exact zeros are dropped, so the pattern itself may differ by cancellation
+    residues), the oracle's own bound on the small cases (test_oracle.test_matrix_invariants).
+    Measured: row sums <= 1.1e-15 and asymmetry <= 1.1e-16 of max|A|; a dropped term or a wrong
+    index breaks them by orders of magnitude more."""
+    m = unit_square(n, delta, jitter=jitter, seed=2024)
+    print(f"[full size] n={n} delta={delta} kernel={kern['function']} nE={m['nE']}", flush=True)
+    a = nlfem.Assembler(m, kern, make_configuration(**ckw))
+    try:
+        A = a.assemble().csr()
+    finally:
+        a.close()
+    scale = abs(A).max()
+    rows = np.where(m["nodeLabels"] > 0)[0]
+    rowsum = np.abs(np.asarray(A[rows].sum(axis=1))).max()
+    AO = A[rows][:, rows].tocsr()
+    D = (AO - AO.T).tocsr()
+    asym = abs(D).max() if D.nnz else 0.0
+    print(f"[full size] nnz={A.nnz} max|A|={scale:.3e} row sums {rowsum / scale:.2e} "
+          f"asymmetry {asym / scale:.2e}", flush=True)
+    assert rowsum <= 1e-13 * scale
+    assert asym <= 1e-13 * scale
