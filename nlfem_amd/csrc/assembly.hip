@@ -2601,11 +2601,13 @@ __global__ __launch_bounds__(256) void k_interior(NumArgs a) {
     const int nF = a.cntF[T0];
     if constexpr (FACT) {
         // Pairs in groups of IU per thread: the group's list entries, then its elements' |det| and
-        // nodes, are loaded before any LDS update, so the dependent loads of a group overlap.
+        // nodes, are loaded before any LDS update, so the dependent loads of a group overlap
+        // (h = 1/256, delta = 0.05: 3.23 ms with 1 pair, 3.11 with 2, 3.22 with 3, 3.15 with 4,
+        // 3.23 with 8, the interior phase with k_merge).
         // Role 0, T as outer element (front segment of its list): column U_j collects w q1_j,
         // row T_i takes p1_i of it below.  Role 1, T as inner element (front entries of its
         // transposed row): -termNonlocPrime into (T_j, U_i), column U_i collects w p1_i.
-        constexpr int IU = 4;
+        constexpr int IU = 2;
         const long long tb = a.tptr[T0];
         const int nT = a.tF[T0];
 #pragma unroll
