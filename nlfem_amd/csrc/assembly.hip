@@ -1076,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_transpose_fill(int nE, int cap, int tg,
         r1 = capT ? r0 + capT : tptr[b + 1];
     };
     auto is_staged_back = [](uint32_t e) { return ((e >> CLS_SHIFT) & 3u) == CLS_CUT; };
-    constexpr int TU = 4;
+    constexpr int TU = 8;  // list loads in flight per thread (4: 2.02 ms, 8: 1.96 ms at h = 1/256)
     // 1: hash, front bits and back counts
     for (int k = 0; k < nk; k++) {
         const int aT = sa[k], nF = sF[k], n = sN[k];
@@ -1287,10 +1287,10 @@ NLFEM_DEV void pattern_row(const PatArgs& a, const int r, int* eh, int* dh) {
         const int ntF = a.tF[T], nt = ntF + a.tB[T];
         if (n + nt > 0 && tid == 0) einsert(T);
         // the entries in groups of PU per thread, their loads issued before the inserts (the
-        // loop is bound by the latency of these loads: 2.63 -> 2.47 ms on the h = 1/256 unit
-        // square with delta = 0.05; 8 per group 2.62 ms)
+        // loop is bound by the latency of these loads; on the h = 1/256 unit square with
+        // delta = 0.05: one per group 2.63 ms, two 2.27, four 2.47, eight 2.62)
         const int tot = n + nt;
-        constexpr int PU = 4;
+        constexpr int PU = 2;
         for (int p0 = tid; p0 < tot; p0 += PU * 256) {
             int Us[PU];
 #pragma unroll
