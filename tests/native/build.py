@@ -3,7 +3,8 @@ under tests/native/_build (run by __graft_entry__.build(); a CPU-only step):
 
 - nlfem_amd_binding.so: the Cython binding of the C ABI that INTEGRATION.md shows (cython
   3, g++), linked against nlfem_amd/libnlfem_amd.so;
-- dropin_main: a C++ program calling the std::string drop-in par_assemble of include/Cassemble.h.
+- dropin_main: a C++ program calling the std::string drop-in par_assemble of include/Cassemble.h,
+  and libdropin.so, the same caller as a shared library (dropin_run) for an in-process call.
 
 Both find the library through an rpath relative to their own location, so the built tree can
 be moved.
@@ -37,6 +38,8 @@ def build():
           "-o", os.path.join(OUT, "nlfem_amd_binding" + ext), "-L", LIBDIR, "-lnlfem_amd", RPATH])
     _run(["g++", "-O2", "-std=c++17", "-I", inc, os.path.join(HERE, "dropin_main.cpp"), "-o",
           os.path.join(OUT, "dropin_main"), "-L", LIBDIR, "-lnlfem_amd", RPATH])
+    _run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-DDROPIN_LIB", "-I", inc, os.path.join(HERE, "dropin_main.cpp"),
+          "-o", os.path.join(OUT, "libdropin.so"), "-L", LIBDIR, "-lnlfem_amd", RPATH])
     return OUT
 
 

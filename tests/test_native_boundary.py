@@ -1,14 +1,19 @@
 """The boundary as compiled callers see it: the Cython binding that INTEGRATION.md gives to
-maintainers (tests/native/Cassemble_amd.pxd, nlfem_amd_binding.pyx) and a C++ program calling
-the std::string drop-in par_assemble of include/Cassemble.h (tests/native/dropin_main.cpp),
-both built by tests/native/build.py (from __graft_entry__.build()).
+maintainers (tests/native/Cassemble_amd.pxd, nlfem_amd_binding.pyx) and a C++ caller of the
+std::string drop-in par_assemble of include/Cassemble.h (tests/native/dropin_main.cpp, built as
+the program dropin_main and as libdropin.so), all built by tests/native/build.py (from
+__graft_entry__.build()).
 
-CPU: both build and link against libnlfem_amd.so; an invalid configuration (dim = 3) is
+CPU: all build and link against libnlfem_amd.so; an invalid configuration (dim = 3) is
 reported as the reference's behaviour dictates: a RuntimeError with the message through the
-Cython binding (status code), abort() with the message through the C++ drop-in.
-GPU: both write the reference's Armadillo file for the C1 mesh, and the file read back equals
-the oracle's matrix (pattern identical, entries within 1e-12 of the row scale).
+Cython binding (status code), abort() with the message through the C++ drop-in (the program,
+in a child process).
+GPU: the binding and the C++ caller (libdropin.so, called in this process: the test starts no
+child process once the GPU is in use) write the reference's Armadillo file for the C1 mesh, and
+the file read back equals the oracle's matrix (pattern identical, entries within 1e-12 of the
+row scale).
 """
+import ctypes
 import importlib
 import os
 import subprocess
@@ -22,10 +27,11 @@ from helpers import case, compare
 HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native", "_build")
 DROPIN = os.path.join(NATIVE, "dropin_main")
+DROPIN_LIB = os.path.join(NATIVE, "libdropin.so")
 
 
 def _built(gpu):
-    have = os.path.exists(DROPIN) and any(f.startswith("nlfem_amd_binding") and f.endswith(".so")
+    have = os.path.exists(DROPIN) and os.path.exists(DROPIN_LIB) and any(f.startswith("nlfem_amd_binding") and f.endswith(".so")
                                           for f in os.listdir(NATIVE)) if os.path.isdir(NATIVE) else False
     if not have:
         if gpu:
@@ -89,9 +95,11 @@ def test_binding_and_dropin_match_oracle(tmp_path):
     inp = tmp_path / "in.bin"
     _write_input(inp, a)
     out = tmp_path / "B.arma"
-    r = subprocess.run([DROPIN, str(inp), str(out), "constant", "retriangulate", "retriangulate"],
-                       capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
+    lib = ctypes.CDLL(DROPIN_LIB)
+    lib.dropin_run.restype = ctypes.c_int
+    lib.dropin_run.argtypes = [ctypes.c_char_p] * 5
+    rc = lib.dropin_run(str(inp).encode(), str(out).encode(), b"constant", b"retriangulate", b"retriangulate")
+    assert rc == 0, rc
     B = nlfem.read_arma_spMat(str(out))
     same, rel, rowrel = compare(B, ref)
     assert same and rowrel <= 1e-12, (same, rowrel)
