@@ -2810,9 +2810,19 @@ __global__ __launch_bounds__(256) void k_merge(int nNodes, int oo, int is_DG, co
         }
         __syncthreads();
         const bool first = c0 == qs;
+        // the partial rows' loads issued eight at a time, summed in row order (h = 1/256:
+        // interior phase with this merge 3.12 -> 3.02 ms against one load per row at a time)
+        const int n = nb;
         for (long long p = threadIdx.x; p < len; p += 256) {
             double v = 0.0;
-            for (int t = 0; t < nb; t++) v += P[base[t] + p];
+            for (int t0 = 0; t0 < n; t0 += 8) {
+                double u[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++) u[t] = t0 + t < n ? P[base[t0 + t] + p] : 0.0;
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (t0 + t < n) v += u[t];
+            }
             if (first) val[r0 * oo + p] = v;
             else val[r0 * oo + p] += v;
         }
